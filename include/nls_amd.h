@@ -1,0 +1,142 @@
+/*
+ * nls_amd.h -- C ABI of libnls_amd.so, the gfx950 HIP implementation of the
+ * time-stepping core of nls (Solver1D/Solver2D.solve -> solve_nls /
+ * solve_nls_2d).
+ *
+ * Every entry point returns an int status: NLS_OK (0) or a negative NLS_E*
+ * code; nls_last_error() gives a message for the calling thread.  No
+ * PyTorch or other framework types appear here: device buffers are plain
+ * pointers, streams are hipStream_t passed as void* (NULL = default stream).
+ *
+ * Data conventions.
+ *   complex arrays: interleaved (re, im) pairs -- complex64 = 2 x float,
+ *     complex128 = 2 x double;
+ *   2-D fields: n*n values, flat index row*n + col with col contiguous.  This
+ *     is byte-identical to the reference's column-major u(n, n) (first index
+ *     contiguous), so arrays pass straight through;
+ *   coeffs: the reference's 23-entry coefficient vector (Fortran c(k) is
+ *     coeffs[k-1]); c3, c4, c5, c6, c11, c12, c13, c14 are read.  c11 == 0
+ *     selects the reference's algebraic (adiabatic) reservoir
+ *     r = c12 P / (c13 + c14 |psi|^2); c11 != 0 integrates the reservoir
+ *     density n as a second field, c11 dn/dt = c12 P - (c13 + c14 |psi|^2) n.
+ */
+#ifndef NLS_AMD_H
+#define NLS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define NLS_OK            0
+#define NLS_EINVAL       -1   /* bad argument (shape, order, null pointer)  */
+#define NLS_EUNSUPPORTED -2   /* valid but unsupported mode combination     */
+#define NLS_EHIP         -3   /* HIP runtime error                          */
+#define NLS_ENONFINITE   -4   /* solution became NaN/Inf                    */
+#define NLS_ENOMEM       -5   /* device allocation failed                   */
+
+/* ---- enums ------------------------------------------------------------- */
+#define NLS_BC_REF       0    /* reference boundaries: 1-D radial r = k*dx with
+                                 a mirror at r = 0 and zero beyond the grid;
+                                 2-D zero outside the grid                  */
+#define NLS_BC_PERIODIC  1    /* Cartesian stencils with wrap-around        */
+#define NLS_F32          0    /* complex64 state, float arithmetic          */
+#define NLS_F64          1    /* complex128 state, double arithmetic        */
+#define NLS_SCHEME_RK4   0    /* classical RK4 over the FD Laplacian        */
+#define NLS_SCHEME_SSF   1    /* Strang split-step Fourier (periodic only)  */
+
+const char *nls_last_error(void);
+
+/* version (nls.f90:29-37): 0.2.0, the reference module version. */
+int nls_version(int *major, int *minor, int *patch);
+
+/* ======================================================================
+ * Drop-in host-memory entry points.  Same argument list and meaning as the
+ * reference's Fortran procedures (scalars by value instead of by
+ * reference); inputs and outputs are host arrays, the work runs on the
+ * current HIP device.  Single precision like the reference.
+ * ====================================================================== */
+
+/* solve_nls (nls/nls.f90:797-813) and solve_nls_1d (:815-827): 1-D
+ * axisymmetric solve, order 3/5/7, iters RK4 steps of dt on r_k = k*dx.
+ * pumping[n] float, coeffs[23] float, u0[n] and u[n] complex64. */
+int nls_solve_nls(float dt, float dx, int n, int order, int iters, const float *pumping,
+                  const float *coeffs, const void *u0, void *u);
+int nls_solve_nls_1d(float dt, float dx, int n, int order, int iters, const float *pumping,
+                     const float *coeffs, const void *u0, void *u);
+
+/* solve_nls_2d (nls/nls.f90:903-919): n x n grid, zero outside. */
+int nls_solve_nls_2d(float dt, float dx, int n, int order, int iters, const float *pumping,
+                     const float *coeffs, const void *u0, void *u);
+
+/* chemical_potential_1d (nls.f90:921-948): mu is complex64 (2 floats). */
+int nls_chemical_potential_1d(float dx, int n, const float *pumping, const float *coeffs,
+                              const void *u0, void *mu);
+/* chemical_potential_2d (nls.f90:950-971): real part only, like the reference. */
+int nls_chemical_potential_2d(float dx, int n, const float *pumping, const float *coeffs,
+                              const void *u0, float *mu);
+
+/* make_laplacian (nls.f90:278-295): the 1-D radial operator in BLAS band
+ * storage, Fortran shape (order, n) -> op[(j-1)*order + (p-1)].  Built in
+ * double precision on the host. */
+int nls_make_laplacian(int n, int order, double h, double *op);
+
+/* ======================================================================
+ * Extended entry points: any precision, boundary family, reservoir mode
+ * and scheme; host or device buffers.
+ * ====================================================================== */
+
+typedef struct nls_problem {
+    int32_t dim;        /* 1 or 2                                          */
+    int32_t n;          /* points per axis                                 */
+    int32_t order;      /* 3, 5 or 7 (stencil width)                       */
+    int32_t bc;         /* NLS_BC_REF / NLS_BC_PERIODIC                    */
+    int32_t dtype;      /* NLS_F32 / NLS_F64                               */
+    int32_t scheme;     /* NLS_SCHEME_RK4 / NLS_SCHEME_SSF                 */
+    double dt, dx;
+    double coeffs[23];
+} nls_problem;
+
+/* Host-memory solve with the reference's semantics and the extensions:
+ * advances psi (complex, dtype) and, when coeffs[10] != 0, the reservoir n
+ * (real, dtype) in place by iters steps.  pumping is real (dtype). */
+int nls_solve_host(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres);
+
+/* Workspace needed by nls_advance for p (bytes). */
+size_t nls_workspace_bytes(const nls_problem *p);
+
+/* Device-memory solve on a stream: psi, nres (dynamic reservoir only) and
+ * pumping are device arrays; psi/nres are advanced in place.  workspace is a
+ * device buffer of at least nls_workspace_bytes(p).  Enqueues work and
+ * returns; the caller synchronises the stream. */
+int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres,
+                void *workspace, size_t workspace_bytes, void *stream);
+
+/* One evaluation of the right-hand side (hamiltonian / hamiltonian_2d,
+ * nls.f90:621-650, :841-870) on device arrays: dpsi = F_psi, dn = F_n
+ * (dn only for the dynamic reservoir). */
+int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const void *nres,
+            void *dpsi, void *dn, void *stream);
+
+/* Chemical potential of a state on device, mu = i <psi, F(psi)>_w / <psi, psi>_w
+ * with the weight r_k = k*dx on the 1-D radial grid and 1 otherwise
+ * (chemical_potential_1d/2d, nls.f90:921-971, generalised to the problem's
+ * order, boundaries and reservoir).  mu[0..1] = (re, im), accumulated in
+ * double precision; blocks until done. */
+int nls_chemical_potential(const nls_problem *p, const void *pumping, const void *psi, const void *nres,
+                           double *mu, void *stream);
+
+/* Observables of a device state, double-precision accumulators in a fixed
+ * order: out[0] = norm  sum |psi|^2 dV, out[1] = population sum n dV,
+ * out[2] = max |psi|^2, out[3] = number of non-finite points.  dV is
+ * 2*pi*r*dr on the 1-D radial grid, dx on a periodic line, dx^2 in 2-D.
+ * Blocks until done. */
+int nls_observables(const nls_problem *p, const void *psi, const void *nres, double *out, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NLS_AMD_H */

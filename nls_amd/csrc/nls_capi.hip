@@ -1,0 +1,295 @@
+// nls_capi.hip -- extern "C" entry points of libnls_amd.so (declared in
+// include/nls_amd.h): argument checking, workspace layout, the RK4 step
+// driver and the host-memory drop-ins of the reference's Fortran
+// procedures.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "nls_amd.h"
+#include "nls_internal.h"
+#include "nls_rk4.h"
+
+namespace nls {
+
+static thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what) {
+    if (e == hipSuccess) return NLS_OK;
+    return fail(e == hipErrorOutOfMemory ? NLS_ENOMEM : NLS_EHIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+int check_problem(const nls_problem *p) {
+    if (!p) return fail(NLS_EINVAL, "problem is NULL");
+    if (p->dim != 1 && p->dim != 2) return fail(NLS_EINVAL, "dim must be 1 or 2 (got %d)", p->dim);
+    if (p->order != 3 && p->order != 5 && p->order != 7)
+        return fail(NLS_EINVAL, "order must be 3, 5 or 7 (got %d)", p->order);
+    if (p->n < p->order) return fail(NLS_EINVAL, "n=%d is smaller than the stencil width %d", p->n, p->order);
+    if (p->dim == 2 && p->n > 46340) return fail(NLS_EINVAL, "n=%d too large for a 2-D grid", p->n);
+    if (p->bc != NLS_BC_REF && p->bc != NLS_BC_PERIODIC) return fail(NLS_EINVAL, "bc must be 0 or 1");
+    if (p->dtype != NLS_F32 && p->dtype != NLS_F64) return fail(NLS_EINVAL, "dtype must be 0 or 1");
+    if (p->scheme != NLS_SCHEME_RK4 && p->scheme != NLS_SCHEME_SSF)
+        return fail(NLS_EINVAL, "scheme must be 0 or 1");
+    if (!(p->dx > 0.0)) return fail(NLS_EINVAL, "dx must be positive");
+    if (p->scheme == NLS_SCHEME_SSF && p->bc != NLS_BC_PERIODIC)
+        return fail(NLS_EUNSUPPORTED, "the split-step Fourier scheme needs periodic boundaries");
+    return NLS_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t points_of(const nls_problem *p) { return p->dim == 1 ? (size_t)p->n : (size_t)p->n * p->n; }
+
+// RK4 workspace: kA, kB, acc (each one complex per point, plus one real for
+// the dynamic reservoir) and, for the 1-D radial operator, its row table.
+static size_t rk4_workspace(const nls_problem *p, size_t *off_tab) {
+    const size_t N = points_of(p), rs = p->dtype == NLS_F64 ? 8 : 4;
+    const bool dyn = p->coeffs[10] != 0.0;
+    size_t b = 3 * align256(N * 2 * rs);
+    if (dyn) b += 3 * align256(N * rs);
+    if (off_tab) *off_tab = b;
+    if (p->dim == 1 && p->bc == NLS_BC_REF) b += align256((size_t)p->n * p->order * rs);
+    return b;
+}
+
+template <typename T>
+static void fill_weights(const nls_problem *p, StageArgs<T> &a) {
+    const int R = (p->order - 1) / 2;
+    T w[4] = {0, 0, 0, 0};
+    for (int d = 0; d <= R; ++d) w[d] = (T)stencil_s2(p->order, d, p->dx);
+    a.w0 = w[0]; a.w1 = w[1]; a.w2 = w[2]; a.w3 = w[3];
+}
+
+template <typename T>
+static int rk4_advance(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws,
+                       hipStream_t s) {
+    using C = cplx_t<T>;
+    const size_t N = points_of(p);
+    const bool dyn = p->coeffs[10] != 0.0;
+    size_t off_tab = 0;
+    rk4_workspace(p, &off_tab);
+    const size_t cb = align256(N * sizeof(C)), rb = align256(N * sizeof(T));
+    C *kA = (C *)ws, *kB = (C *)(ws + cb), *acc = (C *)(ws + 2 * cb);
+    T *kAn = nullptr, *kBn = nullptr, *accn = nullptr;
+    if (dyn) {
+        kAn = (T *)(ws + 3 * cb);
+        kBn = (T *)(ws + 3 * cb + rb);
+        accn = (T *)(ws + 3 * cb + 2 * rb);
+    }
+    StageArgs<T> a;
+    memset(&a, 0, sizeof a);
+    a.n = p->n;
+    a.dt = (T)p->dt;
+    a.c = make_coeffs<T>(p->coeffs);
+    a.P = P;
+    fill_weights<T>(p, a);
+    if (p->dim == 1 && p->bc == NLS_BC_REF) {
+        T *tab = (T *)(ws + off_tab);
+        launch_radial_table<T>(p->n, p->order, p->dx, tab, s);
+        a.wtab = tab;
+    }
+    C *cur = psi, *alt = acc;
+    T *curn = nres, *altn = accn;
+    for (int it = 0; it < iters; ++it) {
+        a.u = cur; a.un = curn; a.acc = alt; a.acc_n = altn;
+        a.stage = 1; a.scale = 0; a.kin = nullptr; a.kin_n = nullptr; a.kout = kA; a.kout_n = kAn;
+        launch_rk4_stage<T>(p->dim, p->order, p->bc, dyn, a, s);
+        a.stage = 2; a.scale = (T)0.5; a.kin = kA; a.kin_n = kAn; a.kout = kB; a.kout_n = kBn;
+        launch_rk4_stage<T>(p->dim, p->order, p->bc, dyn, a, s);
+        a.stage = 3; a.scale = (T)0.5; a.kin = kB; a.kin_n = kBn; a.kout = kA; a.kout_n = kAn;
+        launch_rk4_stage<T>(p->dim, p->order, p->bc, dyn, a, s);
+        a.stage = 4; a.scale = (T)1; a.kin = kA; a.kin_n = kAn; a.kout = nullptr; a.kout_n = nullptr;
+        launch_rk4_stage<T>(p->dim, p->order, p->bc, dyn, a, s);
+        std::swap(cur, alt);
+        std::swap(curn, altn);
+    }
+    int rc = hip_check(hipGetLastError(), "RK4 stage launch");
+    if (rc) return rc;
+    if (cur != psi) {
+        rc = hip_check(hipMemcpyAsync(psi, cur, N * sizeof(C), hipMemcpyDeviceToDevice, s), "copy state");
+        if (rc) return rc;
+        if (dyn) rc = hip_check(hipMemcpyAsync(nres, curn, N * sizeof(T), hipMemcpyDeviceToDevice, s), "copy reservoir");
+    }
+    return rc;
+}
+
+template <typename T>
+static int rhs_once(const nls_problem *p, const T *P, const cplx_t<T> *psi, const T *nres, cplx_t<T> *dpsi,
+                    T *dn, char *ws, hipStream_t s) {
+    const bool dyn = p->coeffs[10] != 0.0;
+    StageArgs<T> a;
+    memset(&a, 0, sizeof a);
+    a.n = p->n;
+    a.dt = (T)p->dt;
+    a.c = make_coeffs<T>(p->coeffs);
+    a.P = P;
+    fill_weights<T>(p, a);
+    if (p->dim == 1 && p->bc == NLS_BC_REF) {
+        launch_radial_table<T>(p->n, p->order, p->dx, (T *)ws, s);
+        a.wtab = (T *)ws;
+    }
+    a.stage = 1; a.u = psi; a.un = nres; a.kout = dpsi; a.kout_n = dn;
+    launch_rk4_stage<T>(p->dim, p->order, p->bc, dyn, a, s);
+    return hip_check(hipGetLastError(), "RHS launch");
+}
+
+// Device scratch owned for the duration of one host-memory call.
+struct DevBuf {
+    void *ptr = nullptr;
+    ~DevBuf() { if (ptr) (void)hipFree(ptr); }
+    int alloc(size_t bytes, const char *what) {
+        return hip_check(hipMalloc(&ptr, bytes ? bytes : 1), what);
+    }
+};
+
+}  // namespace nls
+
+using namespace nls;
+
+extern "C" {
+
+const char *nls_last_error(void) { return g_err; }
+
+int nls_version(int *major, int *minor, int *patch) {
+    if (!major || !minor || !patch) return fail(NLS_EINVAL, "NULL argument");
+    *major = 0; *minor = 2; *patch = 0;
+    return NLS_OK;
+}
+
+size_t nls_workspace_bytes(const nls_problem *p) {
+    if (check_problem(p)) return 0;
+    if (p->scheme == NLS_SCHEME_SSF) return ssf_workspace_bytes(p);
+    return rk4_workspace(p, nullptr);
+}
+
+int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *workspace,
+                size_t workspace_bytes, void *stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (iters < 0) return fail(NLS_EINVAL, "iters must be >= 0");
+    const bool dyn = p->coeffs[10] != 0.0;
+    if (!pumping || !psi || (dyn && !nres)) return fail(NLS_EINVAL, "NULL device buffer");
+    const size_t need = nls_workspace_bytes(p);
+    if (workspace_bytes < need || (need && !workspace))
+        return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+    if (iters == 0) return NLS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (p->scheme == NLS_SCHEME_SSF) return ssf_advance(p, iters, pumping, psi, nres, workspace, s);
+    if (p->dtype == NLS_F64)
+        return rk4_advance<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres,
+                                   (char *)workspace, s);
+    return rk4_advance<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)workspace, s);
+}
+
+int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const void *nres, void *dpsi, void *dn,
+            void *stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    const bool dyn = p->coeffs[10] != 0.0;
+    if (!pumping || !psi || !dpsi || (dyn && (!nres || !dn))) return fail(NLS_EINVAL, "NULL device buffer");
+    hipStream_t s = (hipStream_t)stream;
+    DevBuf tab;
+    if (p->dim == 1 && p->bc == NLS_BC_REF) {
+        rc = tab.alloc((size_t)p->n * p->order * (p->dtype == NLS_F64 ? 8 : 4), "radial table");
+        if (rc) return rc;
+    }
+    if (p->dtype == NLS_F64)
+        rc = rhs_once<double>(p, (const double *)pumping, (const double2 *)psi, (const double *)nres,
+                              (double2 *)dpsi, (double *)dn, (char *)tab.ptr, s);
+    else
+        rc = rhs_once<float>(p, (const float *)pumping, (const float2 *)psi, (const float *)nres, (float2 *)dpsi,
+                             (float *)dn, (char *)tab.ptr, s);
+    if (rc) return rc;
+    // the table must outlive the kernel that reads it
+    return hip_check(hipStreamSynchronize(s), "RHS sync");
+}
+
+int nls_solve_host(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (iters < 0) return fail(NLS_EINVAL, "iters must be >= 0");
+    const bool dyn = p->coeffs[10] != 0.0;
+    if (!pumping || !psi || (dyn && !nres)) return fail(NLS_EINVAL, "NULL host buffer");
+    const size_t N = points_of(p), rs = p->dtype == NLS_F64 ? 8 : 4;
+    DevBuf dP, dpsi, dn, ws;
+    const size_t wsb = nls_workspace_bytes(p);
+    if ((rc = dP.alloc(N * rs, "pumping")) || (rc = dpsi.alloc(N * 2 * rs, "psi")) ||
+        (rc = ws.alloc(wsb, "workspace")))
+        return rc;
+    if (dyn && (rc = dn.alloc(N * rs, "reservoir"))) return rc;
+    if ((rc = hip_check(hipMemcpy(dP.ptr, pumping, N * rs, hipMemcpyHostToDevice), "upload pumping")) ||
+        (rc = hip_check(hipMemcpy(dpsi.ptr, psi, N * 2 * rs, hipMemcpyHostToDevice), "upload psi")))
+        return rc;
+    if (dyn && (rc = hip_check(hipMemcpy(dn.ptr, nres, N * rs, hipMemcpyHostToDevice), "upload reservoir")))
+        return rc;
+    rc = nls_advance(p, iters, dP.ptr, dpsi.ptr, dn.ptr, ws.ptr, wsb, nullptr);
+    if (rc) return rc;
+    if ((rc = hip_check(hipDeviceSynchronize(), "solve")) ||
+        (rc = hip_check(hipMemcpy(psi, dpsi.ptr, N * 2 * rs, hipMemcpyDeviceToHost), "download psi")))
+        return rc;
+    if (dyn) rc = hip_check(hipMemcpy(nres, dn.ptr, N * rs, hipMemcpyDeviceToHost), "download reservoir");
+    return rc;
+}
+
+// ---- drop-ins of the reference's Fortran procedures ----------------------
+
+static int dropin_solve(int dim, float dt, float dx, int n, int order, int iters, const float *pumping,
+                        const float *coeffs, const void *u0, void *u) {
+    if (!pumping || !coeffs || !u0 || !u) return fail(NLS_EINVAL, "NULL argument");
+    nls_problem p;
+    memset(&p, 0, sizeof p);
+    p.dim = dim; p.n = n; p.order = order; p.bc = NLS_BC_REF; p.dtype = NLS_F32; p.scheme = NLS_SCHEME_RK4;
+    p.dt = dt; p.dx = dx;
+    for (int k = 0; k < 23; ++k) p.coeffs[k] = coeffs[k];
+    int rc = check_problem(&p);
+    if (rc) return rc;
+    const size_t N = points_of(&p);
+    std::vector<float> nres;
+    if (p.coeffs[10] != 0.0) nres.assign(N, 0.0f);   // dynamic reservoir starts empty
+    if (u != u0) memcpy(u, u0, N * 2 * sizeof(float));
+    return nls_solve_host(&p, iters, pumping, u, nres.empty() ? nullptr : nres.data());
+}
+
+int nls_solve_nls(float dt, float dx, int n, int order, int iters, const float *pumping, const float *coeffs,
+                  const void *u0, void *u) {
+    return dropin_solve(1, dt, dx, n, order, iters, pumping, coeffs, u0, u);
+}
+
+int nls_solve_nls_1d(float dt, float dx, int n, int order, int iters, const float *pumping, const float *coeffs,
+                     const void *u0, void *u) {
+    return dropin_solve(1, dt, dx, n, order, iters, pumping, coeffs, u0, u);
+}
+
+int nls_solve_nls_2d(float dt, float dx, int n, int order, int iters, const float *pumping, const float *coeffs,
+                     const void *u0, void *u) {
+    return dropin_solve(2, dt, dx, n, order, iters, pumping, coeffs, u0, u);
+}
+
+int nls_make_laplacian(int n, int order, double h, double *op) {
+    if (!op) return fail(NLS_EINVAL, "NULL argument");
+    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7");
+    if (n < order) return fail(NLS_EINVAL, "n smaller than the stencil width");
+    const int R = (order - 1) / 2;
+    std::vector<double> rows((size_t)n * order);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < order; ++k) rows[(size_t)i * order + k] = radial_weight(i, k, n, order, h);
+    // band storage: op(p, j) = A(I, j) with I = p + j - R - 1 (1-based)
+    for (int j = 1; j <= n; ++j)
+        for (int pp = 1; pp <= order; ++pp) {
+            const int I = pp + j - R - 1;
+            op[(size_t)(j - 1) * order + (pp - 1)] =
+                (I >= 1 && I <= n) ? rows[(size_t)(I - 1) * order + (2 * R + 1 - pp)] : 0.0;
+        }
+    return NLS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// nls_core.h -- device-side building blocks shared by the kernels of
+// libnls_amd: complex load/store types, the pointwise right-hand side and
+// the RK4 stage bookkeeping.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "nls_stencil.h"
+
+namespace nls {
+
+enum { BC_REF = 0, BC_PERIODIC = 1 };
+
+template <typename T> struct cvec;
+template <> struct cvec<float> { using type = float2; };
+template <> struct cvec<double> { using type = double2; };
+template <typename T> using cplx_t = typename cvec<T>::type;
+
+template <typename T> __device__ __forceinline__ cplx_t<T> mk(T a, T b) {
+    cplx_t<T> r; r.x = a; r.y = b; return r;
+}
+
+// Coefficients of the model actually read by the core (reference c(k)).
+template <typename T> struct Coeffs {
+    T c3, c4, c5, c6;        // gain, loss, self-interaction, reservoir coupling
+    T c11, c12, c13, c14;    // reservoir: d/dt, pumping, decay, depletion
+};
+
+template <typename T> inline Coeffs<T> make_coeffs(const double *c) {
+    Coeffs<T> r;
+    r.c3 = (T)c[2]; r.c4 = (T)c[3]; r.c5 = (T)c[4]; r.c6 = (T)c[5];
+    r.c11 = (T)c[10]; r.c12 = (T)c[11]; r.c13 = (T)c[12]; r.c14 = (T)c[13];
+    return r;
+}
+
+// Right-hand side at one point, given the stage input (y, yn), pumping P and
+// the Laplacian of y.  Mirrors hamiltonian / revervoir of the reference
+// (nls/nls.f90:570-581, :637-647):
+//   r      = c12 P / (c13 + c14 |y|^2)   (adiabatic)   or yn (dynamic)
+//   k.re   = (c3 r - c4) y.re + (c5 |y|^2 + c6 r) y.im - (L y).im
+//   k.im   = (c3 r - c4) y.im - (c5 |y|^2 + c6 r) y.re + (L y).re
+//   kn     = (c12 P - (c13 + c14 |y|^2) yn) / c11        (dynamic only)
+template <typename T, bool DYN>
+__device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, T yn, cplx_t<T> lap,
+                                          cplx_t<T> &k, T &kn) {
+    const T s = y.x * y.x + y.y * y.y;
+    const T r = DYN ? yn : c.c12 * P / (c.c13 + c.c14 * s);
+    const T g = c.c3 * r - c.c4;
+    const T q = c.c5 * s + c.c6 * r;
+    k.x = g * y.x + q * y.y - lap.y;
+    k.y = g * y.y - q * y.x + lap.x;
+    if (DYN) kn = (c.c12 * P - (c.c13 + c.c14 * s) * yn) / c.c11;
+}
+
+// Periodic index wrap for offsets that may exceed one period.
+__device__ __forceinline__ int wrap_index(int g, int n) {
+    while (g < 0) g += n;
+    while (g >= n) g -= n;
+    return g;
+}
+
+}  // namespace nls

@@ -1,0 +1,273 @@
+"""ctypes boundary to ``libnls_amd.so`` (declared in ``include/nls_amd.h``).
+
+``nls`` mirrors the f2py module object the reference imports as
+``from .native import nls`` (reference ``nls/solver.py:9``): same procedure
+names, positional argument order and optional trailing ``n`` as f2py derives
+them from ``nls/nls.f90`` (e.g. ``u = nls.solve_nls(dt, dx, order, iters,
+pumping, coeffs, u0[, n])``).  Inputs are cast like f2py casts them
+(float32 / complex64, contiguous copies), and a shape mismatch raises
+``ValueError`` as f2py does.  The work runs on the GPU; there is no CPU
+fallback -- a missing library raises ``ImportError``.
+
+The extended entry points (``solve_host``, ``advance``, ``rhs``,
+``observables``, ``chemical_potential``) take an :class:`ProblemDesc` and
+cover double precision, periodic grids, the dynamic reservoir and the
+split-step Fourier scheme.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libnls_amd.so")
+
+NLS_OK, NLS_EINVAL, NLS_EUNSUPPORTED, NLS_EHIP, NLS_ENONFINITE, NLS_ENOMEM = 0, -1, -2, -3, -4, -5
+BC = {"ref": 0, "periodic": 1}
+DTYPE = {"f32": 0, "f64": 1}
+SCHEME = {"rk4": 0, "ssf": 1}
+
+EXPORTS = (
+    "nls_last_error", "nls_version", "nls_solve_nls", "nls_solve_nls_1d", "nls_solve_nls_2d",
+    "nls_chemical_potential_1d", "nls_chemical_potential_2d", "nls_make_laplacian",
+    "nls_solve_host", "nls_workspace_bytes", "nls_advance", "nls_rhs",
+    "nls_chemical_potential", "nls_observables",
+)
+
+
+class NativeError(RuntimeError):
+    """A HIP/runtime failure reported by libnls_amd."""
+
+
+class ProblemDesc(C.Structure):
+    """Mirror of ``nls_problem`` (include/nls_amd.h)."""
+    _fields_ = [("dim", C.c_int32), ("n", C.c_int32), ("order", C.c_int32), ("bc", C.c_int32),
+                ("dtype", C.c_int32), ("scheme", C.c_int32), ("dt", C.c_double), ("dx", C.c_double),
+                ("coeffs", C.c_double * 23)]
+
+    @classmethod
+    def make(cls, dim, n, order, dt, dx, coeffs, bc="ref", dtype="f32", scheme="rk4"):
+        p = cls()
+        p.dim, p.n, p.order = int(dim), int(n), int(order)
+        p.bc = BC[bc] if isinstance(bc, str) else int(bc)
+        p.dtype = DTYPE[dtype] if isinstance(dtype, str) else int(dtype)
+        p.scheme = SCHEME[scheme] if isinstance(scheme, str) else int(scheme)
+        p.dt, p.dx = float(dt), float(dx)
+        c = np.zeros(23)
+        cc = np.asarray(coeffs, np.float64).ravel()
+        c[:min(23, cc.size)] = cc[:23]
+        p.coeffs[:] = list(c)
+        return p
+
+    @property
+    def dynamic(self):
+        return self.coeffs[10] != 0.0
+
+    @property
+    def points(self):
+        return self.n if self.dim == 1 else self.n * self.n
+
+    @property
+    def real_dtype(self):
+        return np.float64 if self.dtype == 1 else np.float32
+
+    @property
+    def complex_dtype(self):
+        return np.complex128 if self.dtype == 1 else np.complex64
+
+    @property
+    def shape(self):
+        return (self.n,) if self.dim == 1 else (self.n, self.n)
+
+
+_lib = None
+
+
+def load():
+    """Load libnls_amd.so; raise ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("libnls_amd.so is not built (python -m nls_amd.build)")
+    L = C.CDLL(LIB_PATH)
+    vp, i, f, d, sz = C.c_void_p, C.c_int, C.c_float, C.c_double, C.c_size_t
+    pp = C.POINTER(ProblemDesc)
+    sig = {
+        "nls_last_error": ([], C.c_char_p),
+        "nls_version": ([vp, vp, vp], i),
+        "nls_solve_nls": ([f, f, i, i, i, vp, vp, vp, vp], i),
+        "nls_solve_nls_1d": ([f, f, i, i, i, vp, vp, vp, vp], i),
+        "nls_solve_nls_2d": ([f, f, i, i, i, vp, vp, vp, vp], i),
+        "nls_chemical_potential_1d": ([f, i, vp, vp, vp, vp], i),
+        "nls_chemical_potential_2d": ([f, i, vp, vp, vp, vp], i),
+        "nls_make_laplacian": ([i, i, d, vp], i),
+        "nls_solve_host": ([pp, i, vp, vp, vp], i),
+        "nls_workspace_bytes": ([pp], sz),
+        "nls_advance": ([pp, i, vp, vp, vp, vp, sz, vp], i),
+        "nls_rhs": ([pp, vp, vp, vp, vp, vp, vp], i),
+        "nls_chemical_potential": ([pp, vp, vp, vp, vp, vp], i),
+        "nls_observables": ([pp, vp, vp, vp, vp], i),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    return L
+
+
+def _check(rc):
+    if rc == NLS_OK:
+        return
+    msg = load().nls_last_error().decode(errors="replace")
+    if rc in (NLS_EINVAL, NLS_EUNSUPPORTED):
+        raise ValueError(msg)
+    if rc == NLS_ENONFINITE:
+        raise FloatingPointError(msg)
+    raise NativeError("libnls_amd error %d: %s" % (rc, msg))
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _arr(x, dtype, name, pos, shape=None):
+    """f2py-style conversion: cast, make contiguous, check the fixed shape."""
+    try:
+        a = np.array(x, dtype=dtype, copy=True, order="C")
+    except (TypeError, ValueError) as e:
+        raise ValueError("failed in converting %s argument `%s' to C/Fortran array: %s" % (pos, name, e))
+    if shape is not None:
+        if a.ndim == 0:
+            a = np.full(shape, a, dtype=dtype)
+        if a.shape != tuple(shape):
+            raise ValueError("%s argument `%s' has shape %s, expected %s" % (pos, name, a.shape, tuple(shape)))
+    return a
+
+
+class _NlsModule(object):
+    """The f2py-shaped namespace ``nls`` (procedures of module nls, nls.f90:13-24)."""
+
+    @staticmethod
+    def version():
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(load().nls_version(C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
+
+    @staticmethod
+    def _solve(fn, dim, dt, dx, order, iters, pumping, coeffs, u0, n):
+        P = _arr(pumping, np.float32, "pumping", "5th")
+        if n is None:
+            n = P.shape[0]
+        shape = (n,) if dim == 1 else (n, n)
+        P = _arr(P, np.float32, "pumping", "5th", shape)
+        co = _arr(coeffs, np.float32, "coeffs", "6th", (23,))
+        u0 = _arr(u0, np.complex64, "u0", "7th", shape)
+        u = np.empty(shape, np.complex64)
+        _check(fn(float(dt), float(dx), int(n), int(order), int(iters), _ptr(P), _ptr(co), _ptr(u0), _ptr(u)))
+        return u
+
+    def solve_nls(self, dt, dx, order, iters, pumping, coeffs, u0, n=None):
+        """u = solve_nls(dt, dx, order, iters, pumping, coeffs, u0[, n]) -- nls.f90:797."""
+        return self._solve(load().nls_solve_nls, 1, dt, dx, order, iters, pumping, coeffs, u0, n)
+
+    def solve_nls_1d(self, dt, dx, order, iters, pumping, coeffs, u0, n=None):
+        """u = solve_nls_1d(...) -- nls.f90:815."""
+        return self._solve(load().nls_solve_nls_1d, 1, dt, dx, order, iters, pumping, coeffs, u0, n)
+
+    def solve_nls_2d(self, dt, dx, order, iters, pumping, coeffs, u0, n=None):
+        """u = solve_nls_2d(dt, dx, order, iters, pumping, coeffs, u0[, n]) -- nls.f90:903."""
+        return self._solve(load().nls_solve_nls_2d, 2, dt, dx, order, iters, pumping, coeffs, u0, n)
+
+    def chemical_potential_1d(self, dx, pumping, coeffs, u0, n=None):
+        """mu = chemical_potential_1d(dx, pumping, coeffs, u0[, n]) -- nls.f90:921 (complex)."""
+        P = _arr(pumping, np.float32, "pumping", "2nd")
+        n = P.shape[0] if n is None else n
+        P = _arr(P, np.float32, "pumping", "2nd", (n,))
+        co = _arr(coeffs, np.float32, "coeffs", "3rd", (23,))
+        u0 = _arr(u0, np.complex64, "u0", "4th", (n,))
+        mu = np.zeros(1, np.complex64)
+        _check(load().nls_chemical_potential_1d(float(dx), int(n), _ptr(P), _ptr(co), _ptr(u0), _ptr(mu)))
+        return mu[0]
+
+    def chemical_potential_2d(self, dx, pumping, coeffs, u0, n=None):
+        """mu = chemical_potential_2d(dx, pumping, coeffs, u0[, n]) -- nls.f90:950 (real)."""
+        P = _arr(pumping, np.float32, "pumping", "2nd")
+        n = P.shape[0] if n is None else n
+        P = _arr(P, np.float32, "pumping", "2nd", (n, n))
+        co = _arr(coeffs, np.float32, "coeffs", "3rd", (23,))
+        u0 = _arr(u0, np.complex64, "u0", "4th", (n, n))
+        mu = np.zeros(1, np.float32)
+        _check(load().nls_chemical_potential_2d(float(dx), int(n), _ptr(P), _ptr(co), _ptr(u0), _ptr(mu)))
+        return mu[0]
+
+    @staticmethod
+    def make_laplacian(n, m, h):
+        """op = make_laplacian(n, m, h) -- nls.f90:278, band (m, n) like the Fortran op(m, n)."""
+        op = np.zeros((int(n), int(m)), np.float64)   # C (n, m) == Fortran (m, n)
+        _check(load().nls_make_laplacian(int(n), int(m), float(h), _ptr(op)))
+        return op.T.astype(np.float32)
+
+
+nls = _NlsModule()
+
+
+# ---------------------------------------------------------------------------
+# Extended entry points
+# ---------------------------------------------------------------------------
+
+def solve_host(desc, iters, pumping, psi0, n0=None):
+    """Host-memory solve (nls_solve_host).  Returns (psi, nres or None)."""
+    rt, ct, shape = desc.real_dtype, desc.complex_dtype, desc.shape
+    P = _arr(pumping, rt, "pumping", "3rd", shape)
+    psi = _arr(psi0, ct, "psi0", "4th", shape)
+    nres = None
+    if desc.dynamic:
+        nres = np.zeros(shape, rt) if n0 is None else _arr(n0, rt, "n0", "5th", shape)
+    _check(load().nls_solve_host(C.byref(desc), int(iters), _ptr(P), _ptr(psi),
+                                 _ptr(nres) if nres is not None else None))
+    return psi, nres
+
+
+def workspace_bytes(desc):
+    return int(load().nls_workspace_bytes(C.byref(desc)))
+
+
+def _dp(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def advance(desc, iters, pumping, psi, nres, workspace, stream=None):
+    """Device solve (nls_advance) on torch tensors; psi/nres advanced in place.
+
+    ``stream`` is a torch.cuda.Stream (default: the current stream)."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    _check(load().nls_advance(C.byref(desc), int(iters), _dp(pumping), _dp(psi), _dp(nres),
+                              _dp(workspace), workspace.numel() * workspace.element_size() if workspace is not None else 0,
+                              C.c_void_p(s.cuda_stream)))
+
+
+def rhs(desc, pumping, psi, nres, dpsi, dn, stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    _check(load().nls_rhs(C.byref(desc), _dp(pumping), _dp(psi), _dp(nres), _dp(dpsi), _dp(dn),
+                          C.c_void_p(s.cuda_stream)))
+
+
+def chemical_potential(desc, pumping, psi, nres=None, stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    mu = (C.c_double * 2)()
+    _check(load().nls_chemical_potential(C.byref(desc), _dp(pumping), _dp(psi), _dp(nres), mu,
+                                         C.c_void_p(s.cuda_stream)))
+    return complex(mu[0], mu[1])
+
+
+def observables(desc, psi, nres=None, stream=None):
+    """(norm, population, peak density, non-finite count) of a device state."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    out = (C.c_double * 4)()
+    _check(load().nls_observables(C.byref(desc), _dp(psi), _dp(nres), out, C.c_void_p(s.cuda_stream)))
+    return {"norm": out[0], "population": out[1], "peak": out[2], "nonfinite": int(out[3])}

@@ -1,0 +1,148 @@
+"""GPU parity of the RK4 finite-difference path against the oracle.
+
+Bar: FP64 on the GPU matches the FP64 oracle to relative L2 <= 1e-10 after
+the full step count; FP32 on the GPU matches the FP64 oracle to <= 1e-4.
+Every call goes through the C ABI of libnls_amd.so.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from nls_amd import configs, native
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-10
+TOL32 = 1e-4
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def small_case(dim, n, bc, dyn, seed=7):
+    rng = np.random.default_rng(seed)
+    if dim == 1:
+        if bc == "ref":
+            x = np.arange(n) * 0.1
+        else:
+            x, _ = configs.grid_1d(n, 12.0)
+        r2 = x ** 2
+        shape = (n,)
+    else:
+        X, Y, _ = configs.grid_2d(n, 8.0)
+        r2 = X ** 2 + Y ** 2
+        shape = (n, n)
+    psi0 = 0.3 * np.exp(-r2 / 4.0) + configs.noise(rng, shape, 1e-2)
+    P = 2.0 * np.exp(-r2 / 6.0)
+    c = configs.model_coeffs(dynamic=dyn)
+    n0 = 0.2 * np.exp(-r2 / 3.0) if dyn else None
+    return P, psi0, n0, c
+
+
+CASES = list(itertools.product((1, 2), ("ref", "periodic"), (False, True), (3, 5, 7)))
+
+
+@pytest.mark.parametrize("dim,bc,dyn,order", CASES)
+def test_rk4_f64_matches_oracle(gpu, oracle, dim, bc, dyn, order):
+    n = 96 if dim == 1 else 40
+    dx = 0.1 if dim == 1 else 0.2
+    P, psi0, n0, c = small_case(dim, n, bc, dyn)
+    steps = 300
+    dt = 2e-3 if dim == 1 else 1e-3
+    if dim == 1 and bc == "ref":
+        dt = 5e-4          # the radial operator's 1/r rows are stiffer
+    desc = native.ProblemDesc.make(dim, n, order, dt, dx, c, bc=bc, dtype="f64")
+    psi, nres = native.solve_host(desc, steps, P, psi0, n0)
+    opsi, onres = oracle.rk4(dim, n, order, 0 if bc == "ref" else 1, dt, dx, steps, P, c, psi0, n0)
+    assert np.all(np.isfinite(psi))
+    assert rel(psi, opsi) <= TOL64, rel(psi, opsi)
+    if dyn:
+        assert rel(nres, onres) <= TOL64
+
+
+@pytest.mark.parametrize("dim,bc,dyn,order", CASES)
+def test_rk4_f32_within_tolerance_of_f64_oracle(gpu, oracle, dim, bc, dyn, order):
+    n = 96 if dim == 1 else 40
+    dx = 0.1 if dim == 1 else 0.2
+    P, psi0, n0, c = small_case(dim, n, bc, dyn)
+    steps = 300
+    dt = 5e-4 if (dim == 1 and bc == "ref") else (2e-3 if dim == 1 else 1e-3)
+    desc = native.ProblemDesc.make(dim, n, order, dt, dx, c, bc=bc, dtype="f32")
+    psi, nres = native.solve_host(desc, steps, P, psi0, n0)
+    opsi, onres = oracle.rk4(dim, n, order, 0 if bc == "ref" else 1, dt, dx, steps, P, c, psi0, n0)
+    assert rel(psi, opsi) <= TOL32, rel(psi, opsi)
+    if dyn:
+        assert rel(nres, onres) <= TOL32
+
+
+def test_dropin_solve_nls_matches_oracle_f32(gpu, oracle):
+    """nls.solve_nls with the reference's argument list (solver.py:66)."""
+    n, dx, dt, steps = 128, 0.1, 1e-3, 500
+    x = np.arange(n) * dx
+    P = 3.0 * np.exp(-x ** 2 / 13.6)
+    c = np.zeros(23)
+    c[2], c[3], c[4], c[5], c[11], c[12], c[13] = 0.5, 0.5, 1.0, 2.0, 0.5, 1.0, 2.0
+    u0 = 0.1 * np.ones(n)
+    u = native.nls.solve_nls(dt, dx, 5, steps, P, c, u0)
+    assert u.dtype == np.complex64 and u.shape == (n,)
+    o32, _ = oracle.rk4(1, n, 5, 0, dt, dx, steps, P, c, u0, dtype=np.float32)
+    o64, _ = oracle.rk4(1, n, 5, 0, dt, dx, steps, P, c, u0)
+    assert rel(u, o64) <= TOL32
+    assert rel(u, o32) <= 1e-5
+
+
+def test_dropin_solve_nls_2d_matches_oracle_f32(gpu, oracle):
+    n, dx, dt, steps = 48, 0.2, 1e-3, 400
+    X, Y, _ = configs.grid_2d(n, n * dx)
+    P = 2.5 * np.exp(-(X ** 2 + Y ** 2) / 10.0)
+    c = configs.model_coeffs(dynamic=False)
+    u0 = 0.1 * np.ones((n, n))
+    u = native.nls.solve_nls_2d(dt, dx, 5, steps, P, c, u0)
+    assert u.dtype == np.complex64 and u.shape == (n, n)
+    o64, _ = oracle.rk4(2, n, 5, 0, dt, dx, steps, P, c, u0)
+    assert rel(u, o64) <= TOL32
+
+
+def test_line256_full_f64(gpu, oracle):
+    """The 1-D workload in full: N=256 periodic, 2000 steps of 1e-3."""
+    inst = configs.line256()
+    desc = native.ProblemDesc.make(1, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic", dtype="f64")
+    psi, nres = native.solve_host(desc, inst.steps, inst.P, inst.psi0, inst.n0)
+    opsi, onres = oracle.rk4(1, inst.n, 5, 1, inst.dt, inst.dx, inst.steps, inst.P, inst.coeffs, inst.psi0, inst.n0)
+    assert rel(psi, opsi) <= TOL64
+    assert rel(nres, onres) <= TOL64
+
+
+def test_square256_f64_first_1000_steps(gpu, oracle):
+    """The 256^2 workload for 1000 of its 10^4 steps (the oracle's share)."""
+    inst = configs.square256()
+    steps = 1000
+    oracle.set_threads(8)
+    desc = native.ProblemDesc.make(2, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic", dtype="f64")
+    psi, nres = native.solve_host(desc, steps, inst.P, inst.psi0, inst.n0)
+    opsi, onres = oracle.rk4(2, inst.n, 5, 1, inst.dt, inst.dx, steps, inst.P, inst.coeffs, inst.psi0, inst.n0)
+    assert rel(psi, opsi) <= TOL64
+    assert rel(nres, onres) <= TOL64
+    desc32 = native.ProblemDesc.make(2, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic", dtype="f32")
+    psi32, _ = native.solve_host(desc32, steps, inst.P, inst.psi0, inst.n0)
+    assert rel(psi32, opsi) <= TOL32
+
+
+@pytest.mark.parametrize("dim", (1, 2))
+def test_dropin_chemical_potential(gpu, oracle, dim):
+    n = 64 if dim == 1 else 32
+    dx = 0.1
+    rng = np.random.default_rng(3)
+    shape = (n,) if dim == 1 else (n, n)
+    u0 = 0.5 + 0.1 * rng.standard_normal(shape) + 0.1j * rng.standard_normal(shape)
+    P = 1.0 + rng.random(shape)
+    c = configs.model_coeffs(dynamic=False)
+    if dim == 1:
+        mu = native.nls.chemical_potential_1d(dx, P, c, u0)
+        ref = oracle.chemical_potential(1, dx, n, P, c, u0)
+    else:
+        mu = native.nls.chemical_potential_2d(dx, P, c, u0)
+        ref = oracle.chemical_potential(2, dx, n, P, c, u0)
+    assert abs(mu - ref) <= 1e-4 * max(1.0, abs(ref))
