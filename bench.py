@@ -1,0 +1,202 @@
+"""Benchmark of the time-stepping core on one MI355X.
+
+Workload (sweep512): 512 independent 2-D instances with sides drawn from
+{64, 128, 256} (p = 0.5/0.3/0.2) on [-16,16)^2, periodic, order-5 stencil,
+ring pumping with per-instance P0 ~ U[0.5,5], r0 ~ U[3,8], w ~ U[0.5,2],
+dynamic reservoir, dt = 5e-4, 5000 RK4 steps, seeded noise per instance
+(nls_amd.configs.sweep512).  A "step" is one RK4 step of the whole batch.
+
+Reported value: grid-point-steps/s of the batched complex64 solve (FP32
+arithmetic) with all inputs resident in HBM.  Beside it: the plain FP64 path
+(one instance at a time) on a sample of the same instances, the relative L2
+error of the batched FP32 result against that FP64 result after the full step
+count, the roofline of the dominant kernel, and the CPU baseline.
+
+  python bench.py [--steps K] [--warmup W] [--gpus 1]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "grid-point-steps/s batched FP32 vs plain FP64 (relL2≤1e-4); p50/p99 latency"
+HBM_PEAK = 8.0e12          # bytes/s, MI355X
+FUSED_BYTES_F32_DYN = 28   # algorithmic bytes per grid-point-step of the fused step kernel: 2 S + r
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def read_traffic(name):
+    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/<name>), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    vals = {}
+    for line in open(path):
+        if "=" in line and not line.lstrip().startswith("#"):
+            k, v = line.split("=", 1)
+            vals[k.strip()] = v.strip()
+    try:
+        return float(vals["traffic_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
+def cpu_baseline(members, steps):
+    """Time the reference Fortran (oracle/_ref, FP32, one core) -- or, where it
+    is not built, the plain-C oracle -- on a bounded sample of the workload."""
+    import ctypes
+    import numpy as np
+    os.environ["MKL_NUM_THREADS"] = "1"
+    from oracle import oracle as O
+    sample = "sweep512 members 0-%d, %d RK4 steps each" % (len(members) - 1, steps)
+    work = sum(m.points for m in members) * steps
+    try:
+        R = O.RefLib(32)
+        try:
+            ctypes.CDLL("libmkl_rt.so.1").MKL_Set_Num_Threads(1)   # the BLAS the reference links
+        except OSError:
+            pass
+        t0 = time.perf_counter()
+        for m in members:
+            R.solve(2, m.dt, m.dx, m.n, 5, steps, m.P, m.coeffs, m.psi0)
+        t = time.perf_counter() - t0
+        return {"value": work / t, "unit": "grid-point-steps/s", "cores": 1, "kind": "reference",
+                "sample": sample + "; reference Fortran solve_nls_2d (FP32, BLAS sgbmv, MKL_NUM_THREADS=1) "
+                          "with its own boundaries and algebraic reservoir (it has no periodic grid or "
+                          "reservoir equation)"}
+    except (OSError, FileNotFoundError):
+        O.set_threads(1)
+        t0 = time.perf_counter()
+        for m in members:
+            O.rk4(2, m.n, 5, 1, m.dt, m.dx, steps, m.P, m.coeffs, m.psi0, m.n0, dtype=np.float32)
+        t = time.perf_counter() - t0
+        return {"value": work / t, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
+                "sample": sample + "; C restatement, FP32, same boundaries and reservoir"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run 5 device-to-device copies of BYTES (calibrates rocprofv3 byte counters)")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py measures one GPU only: every box here has a single MI355X; "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from nls_amd import configs, native
+    from nls_amd.batch import Batch
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    K, W = args.steps, args.warmup
+    if args.copy_calibration:
+        src = torch.ones(args.copy_calibration // 4, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copied %d bytes 5 times" % (src.numel() * 4))
+        return
+
+    t0 = time.perf_counter()
+    members = configs.sweep512(steps=K)
+    points = sum(m.points for m in members)
+    log("sweep512: %d instances, %d points, generated in %.1fs" % (len(members), points, time.perf_counter() - t0))
+
+    batch = Batch(members, dim=2, order=5, bc="periodic", dtype="f32", engine="fused", device=dev)
+    stream = torch.cuda.current_stream()
+    if W > 0:
+        batch.advance(W)
+    torch.cuda.synchronize()
+    batch.reset()
+    torch.cuda.synchronize()
+    log("warmup done (%d steps); timing %d steps" % (W, K))
+
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    w0 = time.perf_counter()
+    e0.record(stream)
+    batch.advance(K)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - w0
+    gpu_s = e0.elapsed_time(e1) * 1e-3
+    value = points * K / wall
+    launch_s = gpu_s / K             # one fused-kernel launch per step; events on its stream
+    log("batched FP32: %.3fs wall, %.1f us/step, %.2f G grid-point-steps/s" % (wall, wall / K * 1e6, value / 1e9))
+
+    bytes_per_launch = points * FUSED_BYTES_F32_DYN
+    achieved = bytes_per_launch / launch_s
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": read_traffic("fused_step_f32_sweep512_traffic.txt"),
+                "kernel": "k_fused_step_2d<float,5,periodic,dynamic>",
+                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6}
+
+    out = {"metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
+           "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic",
+           "config": {"workload": "sweep512: 512 2-D instances, sides 64/128/256 (p=0.5/0.3/0.2), periodic, "
+                                  "order 5, dynamic reservoir, dt=5e-4, %d RK4 steps, complex64 state" % K,
+                      "instances": len(members), "grid_points": points, "engine": "fused",
+                      "parallelism": "single GPU"},
+           "roofline": roofline}
+
+    if not args.no_plain:
+        # plain FP64 path, one instance at a time, on one instance of each size
+        sample = {}
+        for i, m in enumerate(members):
+            if m.n not in sample:
+                sample[m.n] = i
+        rates, errs = {}, {}
+        for n, i in sorted(sample.items()):
+            m = members[i]
+            d = native.ProblemDesc.make(2, n, 5, m.dt, m.dx, m.coeffs, bc="periodic", dtype="f64")
+            P = torch.from_numpy(np.ascontiguousarray(m.P)).to(dev)
+            psi = torch.from_numpy(np.ascontiguousarray(m.psi0, np.complex128)).to(dev)
+            nres = torch.from_numpy(np.ascontiguousarray(m.n0)).to(dev)
+            ws = torch.empty(native.workspace_bytes(d), dtype=torch.uint8, device=dev)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            native.advance(d, K, P, psi, nres, ws)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t1
+            rates[n] = n * n * K / t
+            ref = psi.cpu().numpy()
+            got, _ = batch.member(i)
+            errs[n] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
+            log("plain FP64 %d^2: %.3f G grid-point-steps/s, batched FP32 relL2 = %.2e" % (n, rates[n] / 1e9, errs[n]))
+        # aggregate plain rate over the workload's size mix
+        t_plain = sum(m.points * K / rates[m.n] for m in members)
+        plain = points * K / t_plain
+        out["plain_f64"] = {"value": plain, "unit": "grid-point-steps/s", "per_size": {str(k): v for k, v in rates.items()},
+                            "note": "plain RK4 FP64 path, one instance at a time, rate per size measured on one "
+                                    "instance of each size over the full step count, weighted by the workload mix"}
+        out["speedup_vs_plain_f64"] = value / plain
+        out["accuracy"] = {"max_relL2_vs_f64": max(errs.values()), "per_size": {str(k): v for k, v in errs.items()},
+                           "tolerance": 1e-4}
+
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(members[:16], 200)
+        log("cpu baseline: %.3g grid-point-steps/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["kind"]))
+    batch.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

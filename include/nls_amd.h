@@ -136,6 +136,48 @@ int nls_chemical_potential(const nls_problem *p, const void *pumping, const void
  * Blocks until done. */
 int nls_observables(const nls_problem *p, const void *psi, const void *nres, double *out, void *stream);
 
+/* ======================================================================
+ * Batched solves: many independent instances ("members") of mixed sizes in
+ * shared packed device arrays, advanced together.  Members share dim,
+ * order, boundary family, reservoir mode and dtype; each has its own n, dt,
+ * dx, coefficients and step budget.  No counterpart in the reference, which
+ * runs one solve per worker process (schedulr/workr.py:39-107); each member
+ * follows solve_nls / solve_nls_2d (nls.f90:797, :903).
+ * ====================================================================== */
+
+#define NLS_ENGINE_AUTO  -1   /* FUSED for 2-D, STAGE for 1-D                */
+#define NLS_ENGINE_STAGE  0   /* one launch per RK4 stage                    */
+#define NLS_ENGINE_FUSED  1   /* one launch per RK4 step, halo recomputed on
+                                 chip (2-D only)                             */
+
+typedef struct nls_member {
+    int32_t n;          /* points per axis                                    */
+    int32_t reserved;
+    int64_t offset;     /* index of the member's first point in the packed
+                           psi / n / pumping arrays (elements, not bytes)     */
+    double dt, dx;
+    double coeffs[23];
+} nls_member;
+
+typedef struct nls_batch nls_batch;
+
+/* Plan a batch: validates members (no overlap), builds the tile list and
+ * uploads the member table (device memory owned by the batch). */
+int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nls_member *members, int count,
+                     nls_batch **out);
+int nls_batch_destroy(nls_batch *b);
+/* Length of the packed arrays (points) and number of tiles. */
+size_t nls_batch_points(const nls_batch *b);
+int nls_batch_tiles(const nls_batch *b);
+size_t nls_batch_workspace_bytes(const nls_batch *b);
+/* Step budget per member for the following advance calls (host array of
+ * count entries): member i advances min(iters, steps[i]) steps and then
+ * holds its state.  Default: unlimited.  Synchronises the stream. */
+int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream);
+/* Advance every member by up to iters steps on device arrays in place. */
+int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, void *nres, void *workspace,
+                      size_t workspace_bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,571 @@
+// nls_batch.hip -- batched solves: many independent instances (members) of
+// mixed sizes packed into shared device arrays and advanced together, one
+// launch covering the tiles of every member.
+//
+// Members share dim, order, boundary family, reservoir mode and dtype; each
+// has its own n, dt, dx, coefficients and step budget.  A member whose
+// budget is used up passes its state through unchanged, so a batch can hold
+// solves of different lengths (the serving loop relies on this).
+//
+// Two engines:
+//   NLS_ENGINE_STAGE  one launch per RK4 stage (bodies in nls_stage.h), 15 S
+//                     + 4 r bytes per point and step; any dim/boundary/order.
+//   NLS_ENGINE_FUSED  one launch per RK4 step, 2-D only.  A workgroup loads
+//                     its output tile plus a halo of 4R points (R = stencil
+//                     radius) into registers and LDS once, runs the four
+//                     stages on the shrinking region on chip, and writes the
+//                     new state of the tile: 2 S + r bytes per point and step
+//                     from HBM plus the halo re-reads, at the price of
+//                     recomputing the halo's stages.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "nls_amd.h"
+#include "nls_internal.h"
+#include "nls_rk4.h"
+
+namespace nls {
+
+template <typename T> struct MemberDev {
+    int n, steps;
+    long long off;          // first point of the member in the packed arrays
+    long long tab;          // first entry of its radial table (1-D ref), else -1
+    T dt;
+    T w0, w1, w2, w3;       // Cartesian second-derivative weights
+    Coeffs<T> c;
+};
+
+struct TileDesc {
+    int m, x0, y0, pad;
+};
+
+template <typename T> struct BatchArgs {
+    const MemberDev<T> *mem;
+    const TileDesc *tiles;
+    int ntiles, chunk;      // XCD-aware mapping: blocks of one XCD take a contiguous tile range
+    int it;                 // step index within this advance call
+    int stage;
+    T scale;
+    const T *wtab;
+    const cplx_t<T> *u; const T *un;
+    const cplx_t<T> *kin; const T *kin_n;
+    cplx_t<T> *kout; T *kout_n;
+    cplx_t<T> *acc; T *acc_n;
+    const T *P;
+};
+
+// Workgroups are dispatched to the 8 XCDs round-robin; give each XCD a
+// contiguous range of tiles so neighbouring tiles share its L2.
+__device__ __forceinline__ int xcd_tile(int ntiles, int chunk) {
+    const int b = blockIdx.x;
+    const int t = (b & 7) * chunk + (b >> 3);
+    return t < ntiles ? t : -1;
+}
+
+template <typename T>
+__device__ __forceinline__ StageArgs<T> member_view(const BatchArgs<T> &b, const MemberDev<T> &m) {
+    StageArgs<T> a;
+    a.n = m.n; a.stage = b.stage; a.dt = m.dt; a.scale = b.scale; a.c = m.c;
+    a.w0 = m.w0; a.w1 = m.w1; a.w2 = m.w2; a.w3 = m.w3;
+    a.wtab = m.tab >= 0 ? b.wtab + m.tab : nullptr;
+    const long long o = m.off;
+    a.u = b.u + o; a.un = b.un ? b.un + o : nullptr;
+    a.kin = b.kin ? b.kin + o : nullptr; a.kin_n = b.kin_n ? b.kin_n + o : nullptr;
+    a.kout = b.kout ? b.kout + o : nullptr; a.kout_n = b.kout_n ? b.kout_n + o : nullptr;
+    a.acc = b.acc + o; a.acc_n = b.acc_n ? b.acc_n + o : nullptr;
+    a.P = b.P + o;
+    return a;
+}
+
+// ---------------------------------------------------------------------------
+// Engine STAGE
+// ---------------------------------------------------------------------------
+
+template <typename T, int ORDER, int BC, bool DYN, int DIM>
+__global__ void __launch_bounds__(256) k_batch_stage(const BatchArgs<T> b) {
+    constexpr int R = (ORDER - 1) / 2;
+    constexpr int LDS = DIM == 2 ? (TILE2_Y + 2 * R) * (TILE2_X + 2 * R) : TILE1 + 2 * R;
+    __shared__ cplx_t<T> tile[LDS];
+    const int t = xcd_tile(b.ntiles, b.chunk);
+    if (t < 0) return;
+    const TileDesc td = b.tiles[t];
+    const MemberDev<T> m = b.mem[td.m];
+    const StageArgs<T> a = member_view(b, m);
+    if (b.it >= m.steps) {
+        // budget used up: carry the state into the buffer that becomes u
+        if (b.stage == 4) {
+            const int n = m.n;
+            if (DIM == 2) {
+                const int gx = td.x0 + (threadIdx.x & 63);
+                for (int j = 0; j < TILE2_Y / 4 && gx < n; ++j) {
+                    const int gy = td.y0 + (threadIdx.x >> 6) + 4 * j;
+                    if (gy >= n) break;
+                    const size_t gi = (size_t)gy * n + gx;
+                    a.acc[gi] = a.u[gi];
+                    if (DYN) a.acc_n[gi] = a.un[gi];
+                }
+            } else {
+                const int gx = td.x0 + threadIdx.x;
+                if (gx < n) { a.acc[gx] = a.u[gx]; if (DYN) a.acc_n[gx] = a.un[gx]; }
+            }
+        }
+        return;
+    }
+    if (DIM == 2) stage_tile_2d<T, ORDER, BC, DYN>(a, td.x0, td.y0, tile);
+    else stage_tile_1d<T, ORDER, BC, DYN>(a, td.x0, tile);
+}
+
+// ---------------------------------------------------------------------------
+// Engine FUSED (2-D): one RK4 step per launch, halo recomputation.
+// Output tile 64 x TY, halo H = 4R, region W x HH = (64 + 2H) x (TY + 2H).
+// 256 threads; thread t owns region points t + 256 j.  Per owned point the
+// state u, reservoir, pumping, the stage input of the reservoir and the RK4
+// accumulator stay in registers; the stage input of psi sits in LDS so the
+// stencil can read the neighbours.
+// ---------------------------------------------------------------------------
+
+template <typename T> struct FusedArgs {
+    const MemberDev<T> *mem;
+    const TileDesc *tiles;
+    int ntiles, chunk;
+    int it;
+    const cplx_t<T> *u; const T *un;     // state at the start of the step
+    cplx_t<T> *uo; T *uno;               // state at its end
+    const T *P;
+};
+
+template <typename T> struct FusedTile;
+template <> struct FusedTile<float> { static constexpr int TY = 32; };
+template <> struct FusedTile<double> { static constexpr int TY = 16; };
+
+template <typename T, int ORDER, int BC, bool DYN>
+__global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
+    using C = cplx_t<T>;
+    constexpr int R = (ORDER - 1) / 2, H = 4 * R;
+    constexpr int TX = 64, TY = FusedTile<T>::TY;
+    constexpr int W = TX + 2 * H, HH = TY + 2 * H, NP = W * HH;
+    constexpr int J = (NP + 255) / 256;
+    __shared__ C ys[NP];
+
+    const int t = xcd_tile(f.ntiles, f.chunk);
+    if (t < 0) return;
+    const TileDesc td = f.tiles[t];
+    const MemberDev<T> m = f.mem[td.m];
+    const int n = m.n;
+    const long long off = m.off;
+    const C *u = f.u + off;
+    const T *un = DYN ? f.un + off : nullptr;
+    const T *P = f.P + off;
+    C *uo = f.uo + off;
+    T *uno = DYN ? f.uno + off : nullptr;
+    const int tid = threadIdx.x;
+
+    if (f.it >= m.steps) {            // budget used up: pass the state through
+        for (int j = 0; j < J; ++j) {
+            const int idx = tid + 256 * j;
+            if (idx >= NP) break;
+            const int ly = idx / W, lx = idx - ly * W;
+            if (ly < H || ly >= H + TY || lx < H || lx >= H + TX) continue;
+            const int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
+            if (gx >= n || gy >= n) continue;
+            const size_t gi = (size_t)gy * n + gx;
+            uo[gi] = u[gi];
+            if (DYN) uno[gi] = un[gi];
+        }
+        return;
+    }
+
+    const T dt = m.dt;
+    const Coeffs<T> c = m.c;
+    const T w[4] = {m.w0, m.w1, m.w2, m.w3};
+
+    C uu[J], acc[J], ynext[J];
+    T pp[J], nn[J], yn[J], accn[J];
+    unsigned inside = 0;               // bit j: point j lies on the grid
+
+    // load the state, reservoir and pumping of the whole region once
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int idx = tid + 256 * j;
+        uu[j] = mk<T>(0, 0); pp[j] = 0; nn[j] = 0;
+        acc[j] = mk<T>(0, 0); ynext[j] = mk<T>(0, 0); accn[j] = 0;
+        if (idx < NP) {
+            const int ly = idx / W, lx = idx - ly * W;
+            int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
+            bool ok = true;
+            if (BC == BC_PERIODIC) { gx = wrap_index(gx, n); gy = wrap_index(gy, n); }
+            else ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
+            if (ok) {
+                const size_t gi = (size_t)gy * n + gx;
+                inside |= 1u << j;
+                uu[j] = u[gi];
+                pp[j] = P[gi];
+                if (DYN) nn[j] = un[gi];
+            }
+            ys[idx] = uu[j];
+        }
+        yn[j] = nn[j];
+    }
+    __syncthreads();
+
+    // the four stages on the shrinking region: stage s is valid where the
+    // halo still holds s*R valid rings
+#pragma unroll
+    for (int s = 1; s <= 4; ++s) {
+        const int lo = s * R;
+        const T wacc = (s == 1 || s == 4) ? (T)1 : (T)2;
+        const T anext = (s == 3) ? (T)1 : (T)0.5;  // y_{s+1} = u + (k_s*dt)*anext
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            const int idx = tid + 256 * j;
+            if (idx >= NP) break;
+            const int ly = idx / W, lx = idx - ly * W;
+            if (ly < lo || ly >= HH - lo || lx < lo || lx >= W - lo) continue;
+            const C y = ys[idx];
+            T lr = (T)2 * w[0] * y.x, li = (T)2 * w[0] * y.y;
+#pragma unroll
+            for (int d = 1; d <= R; ++d) {
+                const C a0 = ys[idx - d], a1 = ys[idx + d];
+                const C b0 = ys[idx - d * W], b1 = ys[idx + d * W];
+                lr += w[d] * ((a0.x + a1.x) + (b0.x + b1.x));
+                li += w[d] * ((a0.y + a1.y) + (b0.y + b1.y));
+            }
+            C k;
+            T kn = 0;
+            rhs_point<T, DYN>(c, pp[j], y, yn[j], mk<T>(lr, li), k, kn);
+            if (s == 1) {
+                acc[j] = k;
+                if (DYN) accn[j] = kn;
+            } else {
+                acc[j].x = acc[j].x + wacc * k.x;
+                acc[j].y = acc[j].y + wacc * k.y;
+                if (DYN) accn[j] = accn[j] + wacc * kn;
+            }
+            if (s < 4) {
+                const bool in = (inside >> j) & 1u;
+                ynext[j] = in ? mk<T>(uu[j].x + (k.x * dt) * anext, uu[j].y + (k.y * dt) * anext) : mk<T>(0, 0);
+                if (DYN) yn[j] = nn[j] + (kn * dt) * anext;
+            }
+        }
+        __syncthreads();
+        if (s < 4) {
+            const int lo2 = (s + 1) * R;
+#pragma unroll
+            for (int j = 0; j < J; ++j) {
+                const int idx = tid + 256 * j;
+                if (idx >= NP) break;
+                const int ly = idx / W, lx = idx - ly * W;
+                if (ly < lo2 - R || ly >= HH - lo2 + R || lx < lo2 - R || lx >= W - lo2 + R) continue;
+                ys[idx] = ynext[j];
+            }
+            __syncthreads();
+        }
+    }
+
+    // u_new = u + (k1 + 2 k2 + 2 k3 + k4)*dt/6 on the output tile
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int idx = tid + 256 * j;
+        if (idx >= NP) break;
+        const int ly = idx / W, lx = idx - ly * W;
+        if (ly < H || ly >= H + TY || lx < H || lx >= H + TX) continue;
+        const int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
+        if (gx >= n || gy >= n) continue;
+        const size_t gi = (size_t)gy * n + gx;
+        uo[gi] = mk<T>(uu[j].x + (acc[j].x * dt) / (T)6, uu[j].y + (acc[j].y * dt) / (T)6);
+        if (DYN) uno[gi] = nn[j] + (accn[j] * dt) / (T)6;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+template <typename T, int ORDER, int BC, bool DYN>
+static void launch_batch_stage_t(int dim, const BatchArgs<T> &b, hipStream_t s) {
+    const dim3 grid(8 * b.chunk);
+    if (dim == 2) hipLaunchKernelGGL((k_batch_stage<T, ORDER, BC, DYN, 2>), grid, dim3(256), 0, s, b);
+    else hipLaunchKernelGGL((k_batch_stage<T, ORDER, BC, DYN, 1>), grid, dim3(256), 0, s, b);
+}
+
+template <typename T>
+static void launch_batch_stage(int dim, int order, int bc, bool dyn, const BatchArgs<T> &b, hipStream_t s) {
+#define NLS_CASE(O, B, D) \
+    if (order == O && bc == B && dyn == D) return launch_batch_stage_t<T, O, B, D>(dim, b, s);
+    NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
+    NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
+    NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
+#undef NLS_CASE
+}
+
+template <typename T>
+static void launch_fused(int order, int bc, bool dyn, const FusedArgs<T> &f, hipStream_t s) {
+    const dim3 grid(8 * f.chunk);
+#define NLS_CASE(O, B, D) \
+    if (order == O && bc == B && dyn == D) { \
+        hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D>), grid, dim3(256), 0, s, f); return; }
+    NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
+    NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
+    NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
+#undef NLS_CASE
+}
+
+template <typename T>
+__global__ void k_batch_radial_tables(const MemberDev<T> *mem, int count, int order, const double *dx, T *wtab) {
+    // one workgroup per member, rows strided over its threads
+    const int mi = blockIdx.x;
+    if (mi >= count) return;
+    const MemberDev<T> m = mem[mi];
+    if (m.tab < 0) return;
+    for (int i = threadIdx.x; i < m.n; i += blockDim.x)
+        for (int k = 0; k < order; ++k)
+            wtab[m.tab + (long long)i * order + k] = (T)radial_weight(i, k, m.n, order, dx[mi]);
+}
+
+static size_t align256b(size_t b) { return (b + 255) & ~(size_t)255; }
+
+}  // namespace nls
+
+using namespace nls;
+
+struct nls_batch {
+    int dim, order, bc, dtype, engine;
+    bool dyn;
+    int count;
+    size_t points;                 // length of the packed arrays
+    int ntiles, chunk;
+    void *d_mem = nullptr, *d_tiles = nullptr, *d_wtab = nullptr;
+    void *h_steps_pinned = nullptr;
+    std::vector<unsigned char> h_mem;   // host image of the member table
+    size_t mem_stride;
+};
+
+template <typename T>
+static void fill_members(nls_batch *b, const nls_member *ms, long long *tab_total) {
+    b->mem_stride = sizeof(MemberDev<T>);
+    b->h_mem.assign(sizeof(MemberDev<T>) * b->count, 0);
+    MemberDev<T> *md = (MemberDev<T> *)b->h_mem.data();
+    long long tab = 0;
+    for (int i = 0; i < b->count; ++i) {
+        const nls_member &m = ms[i];
+        MemberDev<T> d;
+        memset(&d, 0, sizeof d);
+        d.n = m.n;
+        d.steps = 0x7fffffff;
+        d.off = m.offset;
+        d.dt = (T)m.dt;
+        d.c = make_coeffs<T>(m.coeffs);
+        const int R = (b->order - 1) / 2;
+        T w[4] = {0, 0, 0, 0};
+        for (int q = 0; q <= R; ++q) w[q] = (T)stencil_s2(b->order, q, m.dx);
+        d.w0 = w[0]; d.w1 = w[1]; d.w2 = w[2]; d.w3 = w[3];
+        d.tab = -1;
+        if (b->dim == 1 && b->bc == NLS_BC_REF) { d.tab = tab; tab += (long long)m.n * b->order; }
+        md[i] = d;
+    }
+    *tab_total = tab;
+}
+
+template <typename T>
+static void set_member_steps(nls_batch *b, int i, int steps) {
+    ((MemberDev<T> *)b->h_mem.data())[i].steps = steps;
+}
+
+template <typename T>
+static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws, hipStream_t s) {
+    using C = cplx_t<T>;
+    const size_t cb = align256b(b->points * sizeof(C)), rb = align256b(b->points * sizeof(T));
+    const bool dyn = b->dyn;
+    C *cur = psi;
+    T *curn = nres;
+    if (b->engine == NLS_ENGINE_FUSED) {
+        C *alt = (C *)ws;
+        T *altn = dyn ? (T *)(ws + cb) : nullptr;
+        FusedArgs<T> f;
+        f.mem = (const MemberDev<T> *)b->d_mem;
+        f.tiles = (const TileDesc *)b->d_tiles;
+        f.ntiles = b->ntiles;
+        f.chunk = b->chunk;
+        f.P = P;
+        for (int it = 0; it < iters; ++it) {
+            f.it = it;
+            f.u = cur; f.un = curn; f.uo = alt; f.uno = altn;
+            launch_fused<T>(b->order, b->bc, dyn, f, s);
+            std::swap(cur, alt);
+            std::swap(curn, altn);
+        }
+    } else {
+        C *kA = (C *)ws, *kB = (C *)(ws + cb), *acc = (C *)(ws + 2 * cb);
+        T *kAn = nullptr, *kBn = nullptr, *accn = nullptr;
+        if (dyn) { kAn = (T *)(ws + 3 * cb); kBn = (T *)(ws + 3 * cb + rb); accn = (T *)(ws + 3 * cb + 2 * rb); }
+        C *alt = acc;
+        T *altn = accn;
+        BatchArgs<T> a;
+        memset(&a, 0, sizeof a);
+        a.mem = (const MemberDev<T> *)b->d_mem;
+        a.tiles = (const TileDesc *)b->d_tiles;
+        a.ntiles = b->ntiles;
+        a.chunk = b->chunk;
+        a.wtab = (const T *)b->d_wtab;
+        a.P = P;
+        for (int it = 0; it < iters; ++it) {
+            a.it = it;
+            a.u = cur; a.un = curn; a.acc = alt; a.acc_n = altn;
+            a.stage = 1; a.scale = 0; a.kin = nullptr; a.kin_n = nullptr; a.kout = kA; a.kout_n = kAn;
+            launch_batch_stage<T>(b->dim, b->order, b->bc, dyn, a, s);
+            a.stage = 2; a.scale = (T)0.5; a.kin = kA; a.kin_n = kAn; a.kout = kB; a.kout_n = kBn;
+            launch_batch_stage<T>(b->dim, b->order, b->bc, dyn, a, s);
+            a.stage = 3; a.scale = (T)0.5; a.kin = kB; a.kin_n = kBn; a.kout = kA; a.kout_n = kAn;
+            launch_batch_stage<T>(b->dim, b->order, b->bc, dyn, a, s);
+            a.stage = 4; a.scale = (T)1; a.kin = kA; a.kin_n = kAn; a.kout = nullptr; a.kout_n = nullptr;
+            launch_batch_stage<T>(b->dim, b->order, b->bc, dyn, a, s);
+            std::swap(cur, alt);
+            std::swap(curn, altn);
+        }
+    }
+    int rc = hip_check(hipGetLastError(), "batch launch");
+    if (rc) return rc;
+    if (cur != psi) {
+        rc = hip_check(hipMemcpyAsync(psi, cur, b->points * sizeof(C), hipMemcpyDeviceToDevice, s), "copy state");
+        if (!rc && dyn)
+            rc = hip_check(hipMemcpyAsync(nres, curn, b->points * sizeof(T), hipMemcpyDeviceToDevice, s), "copy reservoir");
+    }
+    return rc;
+}
+
+extern "C" {
+
+int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nls_member *members, int count,
+                     nls_batch **out) {
+    if (!out || !members || count <= 0) return fail(NLS_EINVAL, "batch needs members and an output handle");
+    *out = nullptr;
+    if (dim != 1 && dim != 2) return fail(NLS_EINVAL, "dim must be 1 or 2");
+    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7");
+    if (bc != NLS_BC_REF && bc != NLS_BC_PERIODIC) return fail(NLS_EINVAL, "bad bc");
+    if (dtype != NLS_F32 && dtype != NLS_F64) return fail(NLS_EINVAL, "bad dtype");
+    if (engine == NLS_ENGINE_AUTO) engine = dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
+    if (engine != NLS_ENGINE_STAGE && engine != NLS_ENGINE_FUSED) return fail(NLS_EINVAL, "bad engine");
+    if (engine == NLS_ENGINE_FUSED && dim != 2) return fail(NLS_EUNSUPPORTED, "the fused engine is 2-D only");
+    const bool dyn = members[0].coeffs[10] != 0.0;
+    size_t points = 0;
+    for (int i = 0; i < count; ++i) {
+        const nls_member &m = members[i];
+        if (m.n < order) return fail(NLS_EINVAL, "member %d: n=%d smaller than the stencil width", i, m.n);
+        if (dim == 2 && m.n > 46340) return fail(NLS_EINVAL, "member %d: n too large", i);
+        if (!(m.dx > 0.0)) return fail(NLS_EINVAL, "member %d: dx must be positive", i);
+        if (m.offset < 0) return fail(NLS_EINVAL, "member %d: negative offset", i);
+        if ((m.coeffs[10] != 0.0) != dyn) return fail(NLS_EINVAL, "members mix reservoir modes");
+        const size_t np = dim == 1 ? (size_t)m.n : (size_t)m.n * m.n;
+        points = std::max(points, (size_t)m.offset + np);
+    }
+    // members must not overlap in the packed arrays
+    {
+        std::vector<std::pair<long long, long long>> iv;
+        for (int i = 0; i < count; ++i) {
+            const long long np = dim == 1 ? members[i].n : (long long)members[i].n * members[i].n;
+            iv.push_back({members[i].offset, members[i].offset + np});
+        }
+        std::sort(iv.begin(), iv.end());
+        for (size_t i = 1; i < iv.size(); ++i)
+            if (iv[i].first < iv[i - 1].second) return fail(NLS_EINVAL, "members overlap in the packed arrays");
+    }
+    nls_batch *b = new nls_batch();
+    b->dim = dim; b->order = order; b->bc = bc; b->dtype = dtype; b->engine = engine; b->dyn = dyn;
+    b->count = count; b->points = points;
+    long long tab_total = 0;
+    if (dtype == NLS_F64) fill_members<double>(b, members, &tab_total);
+    else fill_members<float>(b, members, &tab_total);
+    // tiles, member by member, row-major inside a member
+    std::vector<TileDesc> tiles;
+    int tx = 64, ty = TILE2_Y;
+    if (engine == NLS_ENGINE_FUSED) ty = dtype == NLS_F64 ? FusedTile<double>::TY : FusedTile<float>::TY;
+    if (dim == 1) { tx = TILE1; ty = 1; }
+    for (int i = 0; i < count; ++i) {
+        const int n = members[i].n;
+        const int nyt = dim == 1 ? 1 : (n + ty - 1) / ty, nxt = (n + tx - 1) / tx;
+        for (int y = 0; y < nyt; ++y)
+            for (int x = 0; x < nxt; ++x) tiles.push_back({i, x * tx, dim == 1 ? 0 : y * ty, 0});
+    }
+    b->ntiles = (int)tiles.size();
+    b->chunk = (b->ntiles + 7) / 8;
+    int rc = hip_check(hipMalloc(&b->d_mem, b->h_mem.size()), "batch members");
+    if (!rc) rc = hip_check(hipMalloc(&b->d_tiles, tiles.size() * sizeof(TileDesc)), "batch tiles");
+    if (!rc) rc = hip_check(hipMemcpy(b->d_mem, b->h_mem.data(), b->h_mem.size(), hipMemcpyHostToDevice), "upload");
+    if (!rc) rc = hip_check(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice), "upload");
+    if (!rc && tab_total > 0) {
+        const size_t rs = dtype == NLS_F64 ? 8 : 4;
+        void *ddx = nullptr;
+        std::vector<double> dx(count);
+        for (int i = 0; i < count; ++i) dx[i] = members[i].dx;
+        rc = hip_check(hipMalloc(&b->d_wtab, tab_total * rs), "radial tables");
+        if (!rc) rc = hip_check(hipMalloc(&ddx, count * sizeof(double)), "dx");
+        if (!rc) rc = hip_check(hipMemcpy(ddx, dx.data(), count * sizeof(double), hipMemcpyHostToDevice), "upload");
+        if (!rc) {
+            if (dtype == NLS_F64)
+                hipLaunchKernelGGL(k_batch_radial_tables<double>, dim3(count), dim3(256), 0, 0,
+                                   (const MemberDev<double> *)b->d_mem, count, order, (const double *)ddx, (double *)b->d_wtab);
+            else
+                hipLaunchKernelGGL(k_batch_radial_tables<float>, dim3(count), dim3(256), 0, 0,
+                                   (const MemberDev<float> *)b->d_mem, count, order, (const double *)ddx, (float *)b->d_wtab);
+            rc = hip_check(hipDeviceSynchronize(), "radial tables");
+        }
+        if (ddx) (void)hipFree(ddx);
+    }
+    if (rc) { nls_batch_destroy(b); return rc; }
+    *out = b;
+    return NLS_OK;
+}
+
+int nls_batch_destroy(nls_batch *b) {
+    if (!b) return NLS_OK;
+    if (b->d_mem) (void)hipFree(b->d_mem);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_wtab) (void)hipFree(b->d_wtab);
+    delete b;
+    return NLS_OK;
+}
+
+size_t nls_batch_points(const nls_batch *b) { return b ? b->points : 0; }
+
+int nls_batch_tiles(const nls_batch *b) { return b ? b->ntiles : 0; }
+
+size_t nls_batch_workspace_bytes(const nls_batch *b) {
+    if (!b) return 0;
+    const size_t rs = b->dtype == NLS_F64 ? 8 : 4;
+    const size_t cb = align256b(b->points * 2 * rs), rb = align256b(b->points * rs);
+    if (b->engine == NLS_ENGINE_FUSED) return cb + (b->dyn ? rb : 0);
+    return 3 * cb + (b->dyn ? 3 * rb : 0);
+}
+
+int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream) {
+    if (!b || !steps) return fail(NLS_EINVAL, "NULL argument");
+    for (int i = 0; i < b->count; ++i) {
+        if (steps[i] < 0) return fail(NLS_EINVAL, "member %d: negative step budget", i);
+        if (b->dtype == NLS_F64) set_member_steps<double>(b, i, steps[i]);
+        else set_member_steps<float>(b, i, steps[i]);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    // the member table may still be read by queued launches: wait, then copy
+    int rc = hip_check(hipStreamSynchronize(s), "set_steps sync");
+    if (!rc) rc = hip_check(hipMemcpy(b->d_mem, b->h_mem.data(), b->h_mem.size(), hipMemcpyHostToDevice), "set_steps");
+    return rc;
+}
+
+int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, void *nres, void *workspace,
+                      size_t workspace_bytes, void *stream) {
+    if (!b) return fail(NLS_EINVAL, "NULL batch");
+    if (iters < 0) return fail(NLS_EINVAL, "iters must be >= 0");
+    if (!pumping || !psi || (b->dyn && !nres)) return fail(NLS_EINVAL, "NULL device buffer");
+    const size_t need = nls_batch_workspace_bytes(b);
+    if (workspace_bytes < need || !workspace)
+        return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+    if (iters == 0) return NLS_OK;
+    hipStream_t s = (hipStream_t)stream;
+    if (b->dtype == NLS_F64)
+        return batch_advance<double>(b, iters, (const double *)pumping, (double2 *)psi, (double *)nres,
+                                     (char *)workspace, s);
+    return batch_advance<float>(b, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)workspace, s);
+}
+
+}  // extern "C"

@@ -31,7 +31,10 @@ EXPORTS = (
     "nls_chemical_potential_1d", "nls_chemical_potential_2d", "nls_make_laplacian",
     "nls_solve_host", "nls_workspace_bytes", "nls_advance", "nls_rhs",
     "nls_chemical_potential", "nls_observables",
+    "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
+    "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance",
 )
+ENGINE = {"auto": -1, "stage": 0, "fused": 1}
 
 
 class NativeError(RuntimeError):
@@ -79,6 +82,12 @@ class ProblemDesc(C.Structure):
         return (self.n,) if self.dim == 1 else (self.n, self.n)
 
 
+class Member(C.Structure):
+    """Mirror of ``nls_member`` (include/nls_amd.h)."""
+    _fields_ = [("n", C.c_int32), ("reserved", C.c_int32), ("offset", C.c_int64), ("dt", C.c_double),
+                ("dx", C.c_double), ("coeffs", C.c_double * 23)]
+
+
 _lib = None
 
 
@@ -107,6 +116,13 @@ def load():
         "nls_rhs": ([pp, vp, vp, vp, vp, vp, vp], i),
         "nls_chemical_potential": ([pp, vp, vp, vp, vp, vp], i),
         "nls_observables": ([pp, vp, vp, vp, vp], i),
+        "nls_batch_create": ([i, i, i, i, i, C.POINTER(Member), i, C.POINTER(vp)], i),
+        "nls_batch_destroy": ([vp], i),
+        "nls_batch_points": ([vp], sz),
+        "nls_batch_tiles": ([vp], i),
+        "nls_batch_workspace_bytes": ([vp], sz),
+        "nls_batch_set_steps": ([vp, vp, vp], i),
+        "nls_batch_advance": ([vp, i, vp, vp, vp, vp, sz, vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

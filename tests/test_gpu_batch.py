@@ -1,0 +1,139 @@
+"""GPU parity of the batch engines (nls_batch_* through the C ABI).
+
+Every member of a batch must match the oracle run on that member alone:
+FP64 to relative L2 <= 1e-10, FP32 to <= 1e-4 against the FP64 oracle; a
+member's result must not depend on what else is in the batch (bitwise).
+"""
+import numpy as np
+import pytest
+
+from nls_amd import configs
+from nls_amd.batch import Batch
+
+pytestmark = pytest.mark.gpu
+
+TOL64, TOL32 = 1e-10, 1e-4
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def members_2d(sizes, dyn=True, seed=11, ref=False):
+    out = []
+    for i, n in enumerate(sizes):
+        rng = np.random.default_rng(seed + i)
+        L = 12.0 + 4.0 * (i % 3)
+        X, Y, h = configs.grid_2d(n, L)
+        r = np.sqrt(X ** 2 + Y ** 2)
+        psi0 = 0.2 * np.exp(-r ** 2 / 5.0) + configs.noise(rng, r.shape, 1e-2)
+        P = configs.ring(r, rng.uniform(1.0, 4.0), rng.uniform(1.0, 3.0), rng.uniform(0.5, 1.5))
+        c = configs.model_coeffs(dynamic=dyn, g=rng.uniform(0.5, 1.5))
+        n0 = 0.1 * np.exp(-r ** 2 / 4.0)
+        out.append(configs.Instance("m%d" % i, 2, n, h, rng.choice([5e-4, 1e-3]), 0, P, psi0, n0, c))
+    return out
+
+
+def members_1d(sizes, dyn=True, seed=21, ref=False):
+    out = []
+    for i, n in enumerate(sizes):
+        rng = np.random.default_rng(seed + i)
+        if ref:
+            h = 0.1
+            x = np.arange(n) * h
+        else:
+            x, h = configs.grid_1d(n, 0.1 * n)
+        psi0 = 0.3 * np.exp(-x ** 2 / 4.0) + configs.noise(rng, x.shape, 1e-2)
+        P = 2.0 * np.exp(-x ** 2 / rng.uniform(4.0, 10.0))
+        c = configs.model_coeffs(dynamic=dyn)
+        out.append(configs.Instance("l%d" % i, 1, n, h, 5e-4, 0, P, psi0, 0.1 * np.exp(-x ** 2), c))
+    return out
+
+
+def oracle_member(oracle, m, steps, bc, dim=2, order=5):
+    return oracle.rk4(dim, m.n, order, 0 if bc == "ref" else 1, m.dt, m.dx, steps, m.P, m.coeffs, m.psi0, m.n0)
+
+
+@pytest.mark.parametrize("engine", ("stage", "fused"))
+@pytest.mark.parametrize("bc", ("periodic", "ref"))
+@pytest.mark.parametrize("dyn", (True, False))
+def test_batch_2d_f64_matches_oracle(gpu, oracle, engine, bc, dyn):
+    ms = members_2d([16, 40, 64, 72, 100, 33], dyn=dyn)
+    steps = 150
+    b = Batch(ms, dim=2, order=5, bc=bc, dtype="f64", engine=engine)
+    b.advance(steps)
+    for i, m in enumerate(ms):
+        psi, nres = b.member(i)
+        opsi, onres = oracle_member(oracle, m, steps, bc)
+        assert rel(psi, opsi) <= TOL64, (i, m.n, rel(psi, opsi))
+        if dyn:
+            assert rel(nres, onres) <= TOL64
+    b.close()
+
+
+@pytest.mark.parametrize("order", (3, 7))
+def test_batch_fused_orders(gpu, oracle, order):
+    ms = members_2d([48, 64], dyn=True)
+    b = Batch(ms, dim=2, order=order, bc="periodic", dtype="f64", engine="fused")
+    b.advance(100)
+    for i, m in enumerate(ms):
+        psi, _ = b.member(i)
+        opsi, _ = oracle.rk4(2, m.n, order, 1, m.dt, m.dx, 100, m.P, m.coeffs, m.psi0, m.n0)
+        assert rel(psi, opsi) <= TOL64
+
+
+@pytest.mark.parametrize("bc", ("ref", "periodic"))
+def test_batch_1d_f64_matches_oracle(gpu, oracle, bc):
+    ms = members_1d([40, 256, 300, 1000], ref=(bc == "ref"))
+    b = Batch(ms, dim=1, order=5, bc=bc, dtype="f64", engine="stage")
+    b.advance(200)
+    for i, m in enumerate(ms):
+        psi, nres = b.member(i)
+        opsi, onres = oracle_member(oracle, m, 200, bc, dim=1)
+        assert rel(psi, opsi) <= TOL64, (i, rel(psi, opsi))
+
+
+@pytest.mark.parametrize("engine", ("stage", "fused"))
+def test_step_budgets(gpu, oracle, engine):
+    ms = members_2d([32, 64, 48, 80], dyn=True)
+    budgets = [0, 37, 100, 120]
+    b = Batch(ms, dim=2, order=5, bc="periodic", dtype="f64", engine=engine)
+    b.set_steps(budgets)
+    b.advance(120)
+    for i, m in enumerate(ms):
+        psi, nres = b.member(i)
+        if budgets[i] == 0:
+            np.testing.assert_array_equal(psi, m.psi0)
+            continue
+        opsi, onres = oracle_member(oracle, m, budgets[i], "periodic")
+        assert rel(psi, opsi) <= TOL64, (i, rel(psi, opsi))
+        assert rel(nres, onres) <= TOL64
+
+
+def test_fused_f32_sweep_members_within_tolerance(gpu, oracle):
+    ms = configs.sweep512(count=6, steps=400)
+    b = Batch(ms, dim=2, order=5, bc="periodic", dtype="f32", engine="fused")
+    b.advance(400)
+    oracle.set_threads(8)
+    for i, m in enumerate(ms):
+        psi, nres = b.member(i)
+        opsi, onres = oracle_member(oracle, m, 400, "periodic")
+        assert rel(psi, opsi) <= TOL32, (i, m.n, rel(psi, opsi))
+        assert rel(nres, onres) <= TOL32
+
+
+@pytest.mark.parametrize("engine", ("stage", "fused"))
+def test_member_alone_equals_member_in_mix(gpu, engine):
+    ms = configs.sweep512(count=8, steps=50)
+    mix = Batch(ms, dim=2, order=5, bc="periodic", dtype="f32", engine=engine)
+    mix.set_steps([50, 20, 50, 50, 7, 50, 50, 33])
+    mix.advance(50)
+    for i in (0, 3, 6):
+        alone = Batch([ms[i]], dim=2, order=5, bc="periodic", dtype="f32", engine=engine)
+        alone.advance(50)
+        a, an = alone.member(0)
+        b, bn = mix.member(i)
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(an, bn)
+        alone.close()
+    mix.close()
