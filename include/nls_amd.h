@@ -121,6 +121,12 @@ int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi,
 int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const void *nres,
             void *dpsi, void *dn, void *stream);
 
+/* Tile height (rows: 8, 16 or 32) of the single-instance 2-D RK4 stage kernels
+ * for dtype; rows = 0 only queries.  Returns the height in effect or a
+ * negative NLS_E* code.  Process-wide; set by the autotuner.  No
+ * counterpart in the reference. */
+int nls_tune_stage_tile(int dtype, int rows);
+
 /* Chemical potential of a state on device, mu = i <psi, F(psi)>_w / <psi, psi>_w
  * with the weight r_k = k*dx on the 1-D radial grid and 1 otherwise
  * (chemical_potential_1d/2d, nls.f90:921-971, generalised to the problem's

@@ -27,6 +27,8 @@
 
 namespace nls {
 
+constexpr int BATCH_STAGE_TY = 16;    // tile height of the STAGE engine's 2-D tiles
+
 template <typename T> struct MemberDev {
     int n, steps;
     long long off;          // first point of the member in the packed arrays
@@ -85,7 +87,7 @@ __device__ __forceinline__ StageArgs<T> member_view(const BatchArgs<T> &b, const
 template <typename T, int ORDER, int BC, bool DYN, int DIM>
 __global__ void __launch_bounds__(256) k_batch_stage(const BatchArgs<T> b) {
     constexpr int R = (ORDER - 1) / 2;
-    constexpr int LDS = DIM == 2 ? (TILE2_Y + 2 * R) * (TILE2_X + 2 * R) : TILE1 + 2 * R;
+    constexpr int LDS = DIM == 2 ? (BATCH_STAGE_TY + 2 * R) * (TILE2_X + 2 * R) : TILE1 + 2 * R;
     __shared__ cplx_t<T> tile[LDS];
     const int t = xcd_tile(b.ntiles, b.chunk);
     if (t < 0) return;
@@ -98,7 +100,7 @@ __global__ void __launch_bounds__(256) k_batch_stage(const BatchArgs<T> b) {
             const int n = m.n;
             if (DIM == 2) {
                 const int gx = td.x0 + (threadIdx.x & 63);
-                for (int j = 0; j < TILE2_Y / 4 && gx < n; ++j) {
+                for (int j = 0; j < BATCH_STAGE_TY / 4 && gx < n; ++j) {
                     const int gy = td.y0 + (threadIdx.x >> 6) + 4 * j;
                     if (gy >= n) break;
                     const size_t gi = (size_t)gy * n + gx;
@@ -112,7 +114,7 @@ __global__ void __launch_bounds__(256) k_batch_stage(const BatchArgs<T> b) {
         }
         return;
     }
-    if (DIM == 2) stage_tile_2d<T, ORDER, BC, DYN>(a, td.x0, td.y0, tile);
+    if (DIM == 2) stage_tile_2d<T, ORDER, BC, DYN, BATCH_STAGE_TY>(a, td.x0, td.y0, tile);
     else stage_tile_1d<T, ORDER, BC, DYN>(a, td.x0, tile);
 }
 
@@ -478,7 +480,7 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     else fill_members<float>(b, members, &tab_total);
     // tiles, member by member, row-major inside a member
     std::vector<TileDesc> tiles;
-    int tx = 64, ty = TILE2_Y;
+    int tx = 64, ty = BATCH_STAGE_TY;
     if (engine == NLS_ENGINE_FUSED) ty = dtype == NLS_F64 ? FusedTile<double>::TY : FusedTile<float>::TY;
     if (dim == 1) { tx = TILE1; ty = 1; }
     for (int i = 0; i < count; ++i) {

@@ -274,6 +274,13 @@ int nls_solve_nls_2d(float dt, float dx, int n, int order, int iters, const floa
     return dropin_solve(2, dt, dx, n, order, iters, pumping, coeffs, u0, u);
 }
 
+int nls_tune_stage_tile(int dtype, int rows) {
+    if (dtype != NLS_F32 && dtype != NLS_F64) return fail(NLS_EINVAL, "bad dtype");
+    if (rows == 0) return get_stage_tile(dtype);
+    if (set_stage_tile(dtype, rows)) return fail(NLS_EINVAL, "tile height must be 8, 16 or 32 (got %d)", rows);
+    return rows;
+}
+
 int nls_make_laplacian(int n, int order, double h, double *op) {
     if (!op) return fail(NLS_EINVAL, "NULL argument");
     if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7");

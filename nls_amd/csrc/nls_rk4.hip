@@ -7,11 +7,18 @@
 
 namespace nls {
 
-template <typename T, int ORDER, int BC, bool DYN>
-__global__ void __launch_bounds__(256) k_rk4_stage_2d(const StageArgs<T> a) {
+// Tiles are numbered row-major over the grid; workgroups reach the 8 XCDs
+// round-robin, so block b takes tile (b % 8) * chunk + b / 8 and every XCD
+// sweeps a contiguous band of tile rows (its halo rows stay in its L2).
+template <typename T, int ORDER, int BC, bool DYN, int TY>
+__global__ void __launch_bounds__(256) k_rk4_stage_2d(const StageArgs<T> a, int ntx, int ntiles, int chunk) {
     constexpr int R = (ORDER - 1) / 2;
-    __shared__ cplx_t<T> tile[(TILE2_Y + 2 * R) * (TILE2_X + 2 * R)];
-    stage_tile_2d<T, ORDER, BC, DYN>(a, blockIdx.x * TILE2_X, blockIdx.y * TILE2_Y, tile);
+    __shared__ cplx_t<T> tile[(TY + 2 * R) * (TILE2_X + 2 * R)];
+    const int b = blockIdx.x;
+    const int t = (b & 7) * chunk + (b >> 3);
+    if (t >= ntiles) return;
+    const int ty = t / ntx, tx = t - ty * ntx;
+    stage_tile_2d<T, ORDER, BC, DYN, TY>(a, tx * TILE2_X, ty * TY, tile);
 }
 
 template <typename T, int ORDER, int BC, bool DYN>
@@ -30,11 +37,24 @@ __global__ void k_radial_table(int n, int order, double h, T *wtab) {
     for (int k = 0; k < order; ++k) wtab[(size_t)i * order + k] = (T)radial_weight(i, k, n, order, h);
 }
 
+static int g_stage_ty[2] = {16, 16};      // tile height of the 2-D stage kernels per dtype
+
 template <typename T, int ORDER, int BC, bool DYN>
 static void launch_stage(int dim, const StageArgs<T> &a, hipStream_t s) {
     if (dim == 2) {
-        dim3 grid((a.n + TILE2_X - 1) / TILE2_X, (a.n + TILE2_Y - 1) / TILE2_Y);
-        hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN>), grid, dim3(256), 0, s, a);
+        const int ty = g_stage_ty[sizeof(T) == 8 ? 1 : 0];
+        const int ntx = (a.n + TILE2_X - 1) / TILE2_X;
+        const int nty = (a.n + ty - 1) / ty;
+        const int ntiles = ntx * nty, chunk = (ntiles + 7) / 8;
+        if (ty == 8)
+            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 8>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
+                               ntiles, chunk);
+        else if (ty == 16)
+            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 16>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
+                               ntiles, chunk);
+        else
+            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 32>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
+                               ntiles, chunk);
     } else {
         dim3 grid((a.n + TILE1 - 1) / TILE1);
         hipLaunchKernelGGL((k_rk4_stage_1d<T, ORDER, BC, DYN>), grid, dim3(TILE1), 0, s, a);
@@ -50,6 +70,14 @@ void launch_rk4_stage(int dim, int order, int bc, bool dyn, const StageArgs<T> &
     NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
 #undef NLS_CASE
 }
+
+int set_stage_tile(int dtype, int ty) {
+    if (ty != 8 && ty != 16 && ty != 32) return -1;
+    g_stage_ty[dtype ? 1 : 0] = ty;
+    return 0;
+}
+
+int get_stage_tile(int dtype) { return g_stage_ty[dtype ? 1 : 0]; }
 
 template <typename T>
 void launch_radial_table(int n, int order, double h, T *wtab, hipStream_t s) {

@@ -38,32 +38,43 @@ template <typename T> struct StageArgs {
     const T *P;                  // pumping
 };
 
-// 2-D tile: 64 x 16 points per 256-thread workgroup.
-constexpr int TILE2_X = 64, TILE2_Y = 16;
+// 2-D tiles: 64 x TY points per 256-thread workgroup (TY = 16 or 32).
+constexpr int TILE2_X = 64;
 // 1-D tile: 256 points per 256-thread workgroup.
 constexpr int TILE1 = 256;
 
+// Stage bookkeeping at one point.  The results are formed as values first
+// and written through exactly two store sites with fixed base pointers (k of
+// this stage unless it is the last one; the accumulator unless it is the
+// first); an if-chain with a store per stage let the compiler merge the
+// accumulator stores behind a base-address phi that it left undefined on one
+// path.
 template <typename T, bool DYN>
 __device__ __forceinline__ void stage_epilogue(const StageArgs<T> &a, size_t gi, cplx_t<T> u,
-                                               cplx_t<T> kin, T kin_n, cplx_t<T> k, T kn, T un) {
+                                               cplx_t<T> kin, T kin_n, cplx_t<T> acc, T accn, cplx_t<T> k, T kn,
+                                               T un) {
     const int st = a.stage;
-    if (st == 1) {
+    // acc := k1 + 2 k2 (stage 2), acc + 2 k3 (stage 3), u + (acc + k4) dt/6 (stage 4)
+    const cplx_t<T> base = st == 2 ? kin : acc;
+    const T basen = st == 2 ? kin_n : accn;
+    const T wk = st == 4 ? (T)1 : (T)2;
+    const T sr = base.x + wk * k.x, si = base.y + wk * k.y, sn = basen + wk * kn;
+    cplx_t<T> av;
+    T avn;
+    if (st == 4) {
+        av = mk<T>(u.x + (sr * a.dt) / (T)6, u.y + (si * a.dt) / (T)6);
+        avn = un + (sn * a.dt) / (T)6;
+    } else {
+        av = mk<T>(sr, si);
+        avn = sn;
+    }
+    if (st != 4) {
         a.kout[gi] = k;
         if (DYN) a.kout_n[gi] = kn;
-    } else if (st == 2) {
-        a.kout[gi] = k;
-        a.acc[gi] = mk<T>(kin.x + (T)2 * k.x, kin.y + (T)2 * k.y);
-        if (DYN) { a.kout_n[gi] = kn; a.acc_n[gi] = kin_n + (T)2 * kn; }
-    } else if (st == 3) {
-        a.kout[gi] = k;
-        const cplx_t<T> s = a.acc[gi];
-        a.acc[gi] = mk<T>(s.x + (T)2 * k.x, s.y + (T)2 * k.y);
-        if (DYN) { a.kout_n[gi] = kn; a.acc_n[gi] = a.acc_n[gi] + (T)2 * kn; }
-    } else {
-        const cplx_t<T> s = a.acc[gi];
-        const T sr = s.x + k.x, si = s.y + k.y;
-        a.acc[gi] = mk<T>(u.x + (sr * a.dt) / (T)6, u.y + (si * a.dt) / (T)6);
-        if (DYN) { const T sn = a.acc_n[gi] + kn; a.acc_n[gi] = un + (sn * a.dt) / (T)6; }
+    }
+    if (st != 1) {
+        a.acc[gi] = av;
+        if (DYN) a.acc_n[gi] = avn;
     }
 }
 
@@ -91,53 +102,108 @@ __device__ __forceinline__ void stage_point(const StageArgs<T> &a, size_t gi, cp
         yn = un;
         if (a.stage != 1) { kin_n = a.kin_n[gi]; yn = un + (kin_n * a.dt) * a.scale; }
     }
+    cplx_t<T> acc = mk<T>(0, 0);
+    T accn = 0;
+    if (a.stage >= 3) {
+        acc = a.acc[gi];
+        if (DYN) accn = a.acc_n[gi];
+    }
     cplx_t<T> k;
     T kn = 0;
     rhs_point<T, DYN>(a.c, a.P[gi], y, yn, mk<T>(lr, li), k, kn);
-    stage_epilogue<T, DYN>(a, gi, u, kin, kin_n, k, kn, un);
+    stage_epilogue<T, DYN>(a, gi, u, kin, kin_n, acc, accn, k, kn, un);
 }
 
-// One 64 x 16 tile of a 2-D stage with origin (x0, y0); called by all 256
-// threads of the workgroup.  tile_lds holds (16+2R) x (64+2R) values.
-template <typename T, int ORDER, int BC, bool DYN>
+// One 64 x TY tile of a 2-D stage with origin (x0, y0); called by all 256
+// threads of the workgroup.  Thread (tx, ty) = (tid % 64, tid / 64) owns the
+// TY/4 points (x0 + tx, y0 + ty + 4j).  Every global load of a thread's own
+// points is issued before the LDS barrier, and the own points' stage inputs
+// double as the interior of the LDS tile, so u and k_prev are read once; only
+// the R-wide halo ring is loaded separately.  tile holds (TY+2R) x (64+2R).
+template <typename T, int ORDER, int BC, bool DYN, int TY>
 __device__ __forceinline__ void stage_tile_2d(const StageArgs<T> &a, int x0, int y0, cplx_t<T> *tile) {
+    using C = cplx_t<T>;
     constexpr int R = (ORDER - 1) / 2;
-    constexpr int LW = TILE2_X + 2 * R, LH = TILE2_Y + 2 * R;
+    constexpr int LW = TILE2_X + 2 * R, LH = TY + 2 * R;
+    constexpr int J = TY / 4;
+    constexpr int HALO = 2 * R * TILE2_X + 2 * R * LH;   // top/bottom rows + left/right columns
     const int n = a.n;
     const int tid = threadIdx.x;
-    for (int idx = tid; idx < LW * LH; idx += 256) {
-        const int ly = idx / LW, lx = idx - ly * LW;
-        int gx = x0 + lx - R, gy = y0 + ly - R;
-        cplx_t<T> y = mk<T>(0, 0);
-        if (BC == BC_PERIODIC) {
-            gx = wrap_index(gx, n);
-            gy = wrap_index(gy, n);
-            y = stage_input(a, (size_t)gy * n + gx);
-        } else if (gx >= 0 && gx < n && gy >= 0 && gy < n) {
-            y = stage_input(a, (size_t)gy * n + gx);
+    const int tx = tid & 63, ty = tid >> 6;
+    const bool first = a.stage == 1;
+
+    C u[J], kin[J], acc[J];
+    T un[J], kn_in[J], pp[J], accn[J];
+    size_t gi[J];
+    bool own[J];
+    int gx = x0 + tx;
+    const bool xin = gx < n;
+    if (BC == BC_PERIODIC) gx = wrap_index(gx, n);
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        int gy = y0 + ty + 4 * j;
+        own[j] = xin && gy < n;
+        bool load = own[j];
+        if (BC == BC_PERIODIC) { gy = wrap_index(gy, n); load = true; }
+        else load = load && gy < n;
+        gi[j] = load ? (size_t)gy * n + gx : 0;
+        u[j] = load ? a.u[gi[j]] : mk<T>(0, 0);
+        kin[j] = (load && !first) ? a.kin[gi[j]] : mk<T>(0, 0);
+        un[j] = (DYN && load) ? a.un[gi[j]] : (T)0;
+        kn_in[j] = (DYN && load && !first) ? a.kin_n[gi[j]] : (T)0;
+        pp[j] = own[j] ? a.P[gi[j]] : (T)0;
+        acc[j] = (own[j] && a.stage >= 3) ? a.acc[gi[j]] : mk<T>(0, 0);
+        accn[j] = (DYN && own[j] && a.stage >= 3) ? a.acc_n[gi[j]] : (T)0;
+    }
+    // halo ring
+    for (int h = tid; h < HALO; h += 256) {
+        int lx, ly;
+        if (h < 2 * R * TILE2_X) {                  // rows above and below the tile
+            const int r = h / TILE2_X;
+            lx = R + (h - r * TILE2_X);
+            ly = r < R ? r : TY + r;                 // r in [0,R): top; [R,2R): bottom
+        } else {                                     // columns left and right, full height
+            const int q = h - 2 * R * TILE2_X;
+            const int c = q / LH;
+            ly = q - c * LH;
+            lx = c < R ? c : TILE2_X + c;            // c in [0,R): left; [R,2R): right
         }
-        tile[idx] = y;
+        int hx = x0 + lx - R, hy = y0 + ly - R;
+        C y = mk<T>(0, 0);
+        bool ok = true;
+        if (BC == BC_PERIODIC) { hx = wrap_index(hx, n); hy = wrap_index(hy, n); }
+        else ok = hx >= 0 && hx < n && hy >= 0 && hy < n;
+        if (ok) y = stage_input(a, (size_t)hy * n + hx);
+        tile[ly * LW + lx] = y;
+    }
+    // interior: the own points' stage inputs
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        C y = u[j];
+        if (!first) { y.x = y.x + (kin[j].x * a.dt) * a.scale; y.y = y.y + (kin[j].y * a.dt) * a.scale; }
+        tile[(ty + 4 * j + R) * LW + tx + R] = y;
     }
     __syncthreads();
-    const int tx = tid & 63, ty = tid >> 6;
-    const int gx = x0 + tx;
-    if (gx >= n) return;
     const T w[4] = {a.w0, a.w1, a.w2, a.w3};
 #pragma unroll
-    for (int j = 0; j < TILE2_Y / 4; ++j) {
-        const int gy = y0 + ty + 4 * j;
-        if (gy >= n) break;
+    for (int j = 0; j < J; ++j) {
+        if (!own[j]) continue;
         const int l = (ty + 4 * j + R) * LW + tx + R;
-        const cplx_t<T> c = tile[l];
+        const C c = tile[l];
         T lr = (T)2 * w[0] * c.x, li = (T)2 * w[0] * c.y;
 #pragma unroll
         for (int d = 1; d <= R; ++d) {
-            const cplx_t<T> a0 = tile[l - d], a1 = tile[l + d];
-            const cplx_t<T> b0 = tile[l - d * LW], b1 = tile[l + d * LW];
+            const C a0 = tile[l - d], a1 = tile[l + d];
+            const C b0 = tile[l - d * LW], b1 = tile[l + d * LW];
             lr += w[d] * ((a0.x + a1.x) + (b0.x + b1.x));
             li += w[d] * ((a0.y + a1.y) + (b0.y + b1.y));
         }
-        stage_point<T, DYN>(a, (size_t)gy * n + gx, c, lr, li);
+        T yn = 0;
+        if (DYN) yn = first ? un[j] : un[j] + (kn_in[j] * a.dt) * a.scale;
+        C k;
+        T kn = 0;
+        rhs_point<T, DYN>(a.c, pp[j], c, yn, mk<T>(lr, li), k, kn);
+        stage_epilogue<T, DYN>(a, gi[j], u[j], kin[j], kn_in[j], acc[j], accn[j], k, kn, un[j]);
     }
 }
 

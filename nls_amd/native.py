@@ -32,7 +32,7 @@ EXPORTS = (
     "nls_solve_host", "nls_workspace_bytes", "nls_advance", "nls_rhs",
     "nls_chemical_potential", "nls_observables",
     "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
-    "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance",
+    "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance", "nls_tune_stage_tile",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1}
 
@@ -123,6 +123,7 @@ def load():
         "nls_batch_workspace_bytes": ([vp], sz),
         "nls_batch_set_steps": ([vp, vp, vp], i),
         "nls_batch_advance": ([vp, i, vp, vp, vp, vp, sz, vp], i),
+        "nls_tune_stage_tile": ([i, i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -243,6 +244,14 @@ def solve_host(desc, iters, pumping, psi0, n0=None):
     _check(load().nls_solve_host(C.byref(desc), int(iters), _ptr(P), _ptr(psi),
                                  _ptr(nres) if nres is not None else None))
     return psi, nres
+
+
+def stage_tile(dtype, rows=0):
+    """Get (rows=0) or set the tile height of the 2-D stage kernels for dtype."""
+    r = load().nls_tune_stage_tile(DTYPE[dtype] if isinstance(dtype, str) else int(dtype), int(rows))
+    if r < 0:
+        _check(r)
+    return r
 
 
 def workspace_bytes(desc):
