@@ -42,6 +42,7 @@ int check_problem(const nls_problem *p) {
     if (!(p->dx > 0.0)) return fail(NLS_EINVAL, "dx must be positive");
     if (p->scheme == NLS_SCHEME_SSF && p->bc != NLS_BC_PERIODIC)
         return fail(NLS_EUNSUPPORTED, "the split-step Fourier scheme needs periodic boundaries");
+    if (p->scheme == NLS_SCHEME_SSF) return ssf_check(p);
     return NLS_OK;
 }
 

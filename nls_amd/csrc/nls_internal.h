@@ -14,6 +14,7 @@ size_t points_of(const nls_problem *p);
 
 // split-step Fourier scheme (nls_ssf.hip)
 size_t ssf_workspace_bytes(const nls_problem *p);
+int ssf_check(const nls_problem *p);               // size limits of the scheme
 int ssf_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
                 hipStream_t s);
 

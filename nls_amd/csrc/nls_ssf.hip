@@ -1,12 +1,415 @@
-// nls_ssf.hip -- split-step Fourier scheme (not yet implemented).
+// nls_ssf.hip -- split-step Fourier scheme on periodic grids (no counterpart
+// in the reference, which only has RK4 over finite differences).
+//
+// One step (Strang splitting):  N(dt/2) -> L(dt) -> N(dt/2)
+//   L(dt): psi_hat(k) *= exp(-i |k|^2 dt), the exact flow of psi_t = i lap psi
+//          on the spectral wavenumbers of the grid;
+//   N(t):  one classical RK4 step of size t of the pointwise part of the
+//          reference's right-hand side (nls/nls.f90:570-581, :637-644) plus
+//          the reservoir equation when c11 != 0.
+//
+// 2-D: the state between passes is kept transformed along x ("row space"):
+//   row pass  : inverse x-FFT -> N(dt/2) [end of step] -> N(dt/2) [start of
+//               the next] -> forward x-FFT           (reads/writes psi, n, P)
+//   column pass: forward y-FFT -> phase exp(-i (kx^2 + ky^2) dt)/n^2 ->
+//               inverse y-FFT                          (reads/writes psi)
+// so a step moves 4 c + 3 r bytes per point (c complex, r real).  The first
+// row pass has no inverse transform and the last one no forward transform.
+// 1-D: a workgroup holds a whole line (n <= 4096) in LDS and runs all steps
+// on chip.
+//
+// FFTs: in-place Stockham radix-4 (last stage radix-2 when log2 n is odd) on
+// up to 4096 complex values per workgroup held in LDS; each stage stages its
+// butterfly inputs in registers, synchronises, then writes the outputs in
+// place.  Twiddles exp(-2 pi i m / n) and the phase table are generated in
+// double precision and rounded once to the working precision.
+#include <cstring>
+
+#include "nls_core.h"
 #include "nls_internal.h"
 
 namespace nls {
 
-size_t ssf_workspace_bytes(const nls_problem *) { return 0; }
+constexpr int SSF_PTS = 4096;       // complex values per workgroup
+constexpr int SSF_LOG_PTS = 12;
 
-int ssf_advance(const nls_problem *, int, const void *, void *, void *, void *, hipStream_t) {
-    return fail(NLS_EUNSUPPORTED, "split-step Fourier scheme not built");
+template <typename T>
+__device__ __forceinline__ cplx_t<T> cmul(cplx_t<T> a, cplx_t<T> b) {
+    return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// Element m of line l lives at buf[l*lstride + m*estride].  LINE_FAST maps
+// consecutive threads to consecutive lines (column pass: lines are adjacent
+// in LDS) instead of consecutive elements (row pass).
+template <typename T, bool INV, bool LINE_FAST>
+__device__ __forceinline__ void fft_radix4_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+                                                 int logNs, const cplx_t<T> *tw) {
+    using C = cplx_t<T>;
+    const int lognb = logN - 2;
+    const int total = 1 << (logL + lognb);      // butterflies in the workgroup (<= 1024)
+    const int Ns = 1 << logNs;
+    const int sh = logN - logNs - 2;
+    C v[4][4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int b = threadIdx.x + 256 * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[q][r] = buf[l * lstride + (j + (r << lognb)) * estride];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int b = threadIdx.x + 256 * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < 4; ++r) {
+                C w = tw[(r * k) << sh];
+                if (INV) w.y = -w.y;
+                v[q][r] = cmul<T>(v[q][r], w);
+            }
+            const C t0 = mk<T>(v[q][0].x + v[q][2].x, v[q][0].y + v[q][2].y);
+            const C t1 = mk<T>(v[q][0].x - v[q][2].x, v[q][0].y - v[q][2].y);
+            const C t2 = mk<T>(v[q][1].x + v[q][3].x, v[q][1].y + v[q][3].y);
+            const C d13 = mk<T>(v[q][1].x - v[q][3].x, v[q][1].y - v[q][3].y);
+            // forward: (a1 - a3) * (-i); inverse: * (+i)
+            const C t3 = INV ? mk<T>(-d13.y, d13.x) : mk<T>(d13.y, -d13.x);
+            const int d = ((j >> logNs) << (logNs + 2)) + k;
+            const int o = l * lstride;
+            buf[o + (d) * estride] = mk<T>(t0.x + t2.x, t0.y + t2.y);
+            buf[o + (d + Ns) * estride] = mk<T>(t1.x + t3.x, t1.y + t3.y);
+            buf[o + (d + 2 * Ns) * estride] = mk<T>(t0.x - t2.x, t0.y - t2.y);
+            buf[o + (d + 3 * Ns) * estride] = mk<T>(t1.x - t3.x, t1.y - t3.y);
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, bool INV, bool LINE_FAST>
+__device__ __forceinline__ void fft_radix2_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+                                                 int logNs, const cplx_t<T> *tw) {
+    using C = cplx_t<T>;
+    const int lognb = logN - 1;
+    const int total = 1 << (logL + lognb);      // <= 2048
+    const int Ns = 1 << logNs;
+    const int sh = logN - logNs - 1;
+    C v[8][2];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int b = threadIdx.x + 256 * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            v[q][0] = buf[l * lstride + j * estride];
+            v[q][1] = buf[l * lstride + (j + (1 << lognb)) * estride];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+        const int b = threadIdx.x + 256 * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+            C w = tw[k << sh];
+            if (INV) w.y = -w.y;
+            const C a1 = cmul<T>(v[q][1], w);
+            const int d = ((j >> logNs) << (logNs + 1)) + k;
+            const int o = l * lstride;
+            buf[o + d * estride] = mk<T>(v[q][0].x + a1.x, v[q][0].y + a1.y);
+            buf[o + (d + Ns) * estride] = mk<T>(v[q][0].x - a1.x, v[q][0].y - a1.y);
+        }
+    }
+    __syncthreads();
+}
+
+// In-place FFT of 2^logL lines of length 2^logN (2^(logL+logN) <= 4096) in LDS.
+template <typename T, bool INV, bool LINE_FAST>
+__device__ __forceinline__ void lds_fft(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+                                        const cplx_t<T> *tw) {
+    int logNs = 0;
+    while (logN - logNs >= 2) {
+        fft_radix4_stage<T, INV, LINE_FAST>(buf, logN, logL, lstride, estride, logNs, tw);
+        logNs += 2;
+    }
+    if (logNs < logN) fft_radix2_stage<T, INV, LINE_FAST>(buf, logN, logL, lstride, estride, logNs, tw);
+}
+
+// One RK4 step of size tau of the pointwise system at one point.
+template <typename T, bool DYN>
+__device__ __forceinline__ void pointwise_rk4(const Coeffs<T> &c, T inv_c11, T P, T tau, cplx_t<T> &psi, T &n) {
+    auto f = [&](cplx_t<T> y, T yn, cplx_t<T> &k, T &kn) {
+        const T s = y.x * y.x + y.y * y.y;
+        const T r = DYN ? yn : c.c12 * P / (c.c13 + c.c14 * s);
+        const T g = c.c3 * r - c.c4, q = c.c5 * s + c.c6 * r;
+        k = mk<T>(g * y.x + q * y.y, g * y.y - q * y.x);
+        kn = DYN ? (c.c12 * P - (c.c13 + c.c14 * s) * yn) * inv_c11 : (T)0;
+    };
+    cplx_t<T> k1, k2, k3, k4;
+    T m1, m2, m3, m4;
+    const T h = tau * (T)0.5;
+    f(psi, n, k1, m1);
+    f(mk<T>(psi.x + k1.x * h, psi.y + k1.y * h), n + m1 * h, k2, m2);
+    f(mk<T>(psi.x + k2.x * h, psi.y + k2.y * h), n + m2 * h, k3, m3);
+    f(mk<T>(psi.x + k3.x * tau, psi.y + k3.y * tau), n + m3 * tau, k4, m4);
+    const T t6 = tau / (T)6;
+    psi = mk<T>(psi.x + (k1.x + (T)2 * k2.x + (T)2 * k3.x + k4.x) * t6,
+                psi.y + (k1.y + (T)2 * k2.y + (T)2 * k3.y + k4.y) * t6);
+    if (DYN) n = n + (m1 + (T)2 * m2 + (T)2 * m3 + m4) * t6;
+}
+
+template <typename T> struct SsfArgs {
+    int logN;
+    T half_dt, inv_c11;
+    Coeffs<T> c;
+    cplx_t<T> *psi;
+    T *nres;
+    const T *P;
+    const cplx_t<T> *tw;      // exp(-2 pi i m / N)
+    const cplx_t<T> *ph;      // exp(-i k_m^2 dt) / N
+    int iters;                // 1-D kernel only
+};
+
+enum { ROWS_FIRST = 0, ROWS_MIDDLE = 1, ROWS_LAST = 2 };
+
+// Row pass.  2^(12-logN) rows per workgroup (at most n).
+template <typename T, bool DYN, int MODE>
+__global__ void __launch_bounds__(256) k_ssf_rows(const SsfArgs<T> a) {
+    using C = cplx_t<T>;
+    __shared__ C buf[SSF_PTS];
+    const int logN = a.logN, N = 1 << logN;
+    int logL = SSF_LOG_PTS - logN;
+    if (logL > logN) logL = logN;
+    const int pts = 1 << (logL + logN);
+    const size_t base = (size_t)blockIdx.x << (logL + logN);    // rows are contiguous
+    T pp[16], nn[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        pp[q] = 0; nn[q] = 0;
+        if (idx < pts) {
+            buf[idx] = a.psi[base + idx];
+            pp[q] = a.P[base + idx];
+            if (DYN) nn[q] = a.nres[base + idx];
+        }
+    }
+    __syncthreads();
+    if (MODE != ROWS_FIRST) lds_fft<T, true, false>(buf, logN, logL, N, 1, a.tw);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        if (idx < pts) {
+            C v = buf[idx];
+            T nv = nn[q];
+            pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[q], a.half_dt, v, nv);
+            if (MODE == ROWS_MIDDLE) pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[q], a.half_dt, v, nv);
+            buf[idx] = v;
+            nn[q] = nv;
+        }
+    }
+    __syncthreads();
+    if (MODE != ROWS_LAST) lds_fft<T, false, false>(buf, logN, logL, N, 1, a.tw);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        if (idx < pts) {
+            a.psi[base + idx] = buf[idx];
+            if (DYN) a.nres[base + idx] = nn[q];
+        }
+    }
+}
+
+// Column pass: 2^(12-logN) adjacent columns per workgroup, LDS [row][col].
+// Workgroups of one XCD take adjacent column blocks (round-robin dispatch).
+template <typename T>
+__global__ void __launch_bounds__(256) k_ssf_cols(const SsfArgs<T> a, int nblocks, int chunk) {
+    using C = cplx_t<T>;
+    __shared__ C buf[SSF_PTS];
+    const int b = blockIdx.x;
+    const int blk = (b & 7) * chunk + (b >> 3);
+    if (blk >= nblocks) return;
+    const int logN = a.logN, N = 1 << logN;
+    int logL = SSF_LOG_PTS - logN;
+    if (logL > logN) logL = logN;
+    const int L = 1 << logL, pts = L << logN;
+    const int col0 = blk << logL;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        if (idx < pts) {
+            const int m = idx >> logL, cc = idx & (L - 1);
+            buf[idx] = a.psi[(size_t)m * N + col0 + cc];
+        }
+    }
+    __syncthreads();
+    lds_fft<T, false, true>(buf, logN, logL, 1, L, a.tw);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        if (idx < pts) {
+            const int m = idx >> logL, cc = idx & (L - 1);
+            buf[idx] = cmul<T>(buf[idx], cmul<T>(a.ph[m], a.ph[col0 + cc]));
+        }
+    }
+    __syncthreads();
+    lds_fft<T, true, true>(buf, logN, logL, 1, L, a.tw);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        if (idx < pts) {
+            const int m = idx >> logL, cc = idx & (L - 1);
+            a.psi[(size_t)m * N + col0 + cc] = buf[idx];
+        }
+    }
+}
+
+// 1-D: one workgroup holds the whole line in LDS and runs every step; the
+// pumping and reservoir of the thread's points stay in registers.
+template <typename T, bool DYN>
+__global__ void __launch_bounds__(256) k_ssf_line(const SsfArgs<T> a) {
+    using C = cplx_t<T>;
+    __shared__ C buf[SSF_PTS];
+    const int logN = a.logN, N = 1 << logN;
+    T pp[16], nn[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        pp[q] = 0; nn[q] = 0;
+        if (idx < N) {
+            buf[idx] = a.psi[idx];
+            pp[q] = a.P[idx];
+            if (DYN) nn[q] = a.nres[idx];
+        }
+    }
+    for (int it = 0; it < a.iters; ++it) {
+        // each thread only touches its own points here, so no barrier is
+        // needed before the first FFT stage (which starts with one)
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int idx = threadIdx.x + 256 * q;
+            if (idx < N) {
+                C v = buf[idx];
+                pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[q], a.half_dt, v, nn[q]);
+                buf[idx] = v;
+            }
+        }
+        __syncthreads();
+        lds_fft<T, false, false>(buf, logN, 0, N, 1, a.tw);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int idx = threadIdx.x + 256 * q;
+            if (idx < N) buf[idx] = cmul<T>(buf[idx], a.ph[idx]);
+        }
+        __syncthreads();
+        lds_fft<T, true, false>(buf, logN, 0, N, 1, a.tw);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) {
+            const int idx = threadIdx.x + 256 * q;
+            if (idx < N) {
+                C v = buf[idx];
+                pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[q], a.half_dt, v, nn[q]);
+                buf[idx] = v;
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const int idx = threadIdx.x + 256 * q;
+        if (idx < N) {
+            a.psi[idx] = buf[idx];
+            if (DYN) a.nres[idx] = nn[q];
+        }
+    }
+}
+
+// Twiddles and phase, double precision, rounded once to T.
+template <typename T>
+__global__ void k_ssf_tables(int N, double dx, double dt, double scale, cplx_t<T> *tw, cplx_t<T> *ph) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m >= N) return;
+    double s, c;
+    sincospi(2.0 * (double)m / (double)N, &s, &c);
+    tw[m] = mk<T>((T)c, (T)-s);
+    const int f = m < N / 2 ? m : m - N;                 // FFT frequency order
+    const double k = 2.0 * M_PI * (double)f / ((double)N * dx);
+    double ps, pc;
+    sincos(k * k * dt, &ps, &pc);
+    ph[m] = mk<T>((T)(pc * scale), (T)(-ps * scale));
+}
+
+static size_t align256s(size_t b) { return (b + 255) & ~(size_t)255; }
+
+static int log2_exact(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return (1 << l) == n ? l : -1;
+}
+
+template <typename T, bool DYN>
+static int ssf_run(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws, hipStream_t s) {
+    using C = cplx_t<T>;
+    const int N = p->n, logN = log2_exact(N);
+    C *tw = (C *)ws, *ph = (C *)(ws + align256s(N * sizeof(C)));
+    // 1/N per inverse transform: once in 1-D, once per axis (folded into the
+    // product ph[ky] * ph[kx]) in 2-D
+    hipLaunchKernelGGL(k_ssf_tables<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, p->dx, p->dt, 1.0 / N, tw, ph);
+    SsfArgs<T> a;
+    memset(&a, 0, sizeof a);
+    a.logN = logN;
+    a.half_dt = (T)(0.5 * p->dt);
+    a.c = make_coeffs<T>(p->coeffs);
+    a.inv_c11 = DYN ? (T)(1.0 / p->coeffs[10]) : (T)0;
+    a.psi = psi; a.nres = nres; a.P = P; a.tw = tw; a.ph = ph;
+    if (p->dim == 1) {
+        a.iters = iters;
+        hipLaunchKernelGGL((k_ssf_line<T, DYN>), dim3(1), dim3(256), 0, s, a);
+        return hip_check(hipGetLastError(), "ssf line launch");
+    }
+    int logL = SSF_LOG_PTS - logN;
+    if (logL > logN) logL = logN;
+    const int blocks = N >> logL;
+    const int chunk = (blocks + 7) / 8;
+    hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_FIRST>), dim3(blocks), dim3(256), 0, s, a);
+    for (int it = 0; it < iters; ++it) {
+        hipLaunchKernelGGL(k_ssf_cols<T>, dim3(8 * chunk), dim3(256), 0, s, a, blocks, chunk);
+        if (it + 1 < iters) hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_MIDDLE>), dim3(blocks), dim3(256), 0, s, a);
+        else hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_LAST>), dim3(blocks), dim3(256), 0, s, a);
+    }
+    return hip_check(hipGetLastError(), "ssf launch");
+}
+
+size_t ssf_workspace_bytes(const nls_problem *p) {
+    const size_t cs = p->dtype == NLS_F64 ? 16 : 8;
+    return 2 * align256s((size_t)p->n * cs);
+}
+
+int ssf_check(const nls_problem *p) {
+    const int l = log2_exact(p->n);
+    const int maxn = p->dim == 1 ? 4096 : 2048;
+    if (l < 0 || p->n < 16 || p->n > maxn)
+        return fail(NLS_EUNSUPPORTED, "split-step Fourier needs n a power of two in [16, %d] (got %d)", maxn, p->n);
+    return NLS_OK;
+}
+
+int ssf_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
+                hipStream_t s) {
+    int rc = ssf_check(p);
+    if (rc) return rc;
+    const bool dyn = p->coeffs[10] != 0.0;
+    if (p->dtype == NLS_F64) {
+        if (dyn) return ssf_run<double, true>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)ws, s);
+        return ssf_run<double, false>(p, iters, (const double *)pumping, (double2 *)psi, nullptr, (char *)ws, s);
+    }
+    if (dyn) return ssf_run<float, true>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);
+    return ssf_run<float, false>(p, iters, (const float *)pumping, (float2 *)psi, nullptr, (char *)ws, s);
 }
 
 }  // namespace nls
