@@ -151,10 +151,14 @@ int nls_observables(const nls_problem *p, const void *psi, const void *nres, dou
  * follows solve_nls / solve_nls_2d (nls.f90:797, :903).
  * ====================================================================== */
 
-#define NLS_ENGINE_AUTO  -1   /* FUSED for 2-D, STAGE for 1-D                */
+#define NLS_ENGINE_AUTO  -1   /* RESIDENT if every member fits, else FUSED for
+                                 2-D and STAGE for 1-D                       */
 #define NLS_ENGINE_STAGE  0   /* one launch per RK4 stage                    */
 #define NLS_ENGINE_FUSED  1   /* one launch per RK4 step, halo recomputed on
                                  chip (2-D only)                             */
+#define NLS_ENGINE_RESIDENT 2 /* one launch per advance call: a workgroup per
+                                 member holds its grid on chip and runs every
+                                 step (1-D n <= 4096, 2-D n <= 64)           */
 
 typedef struct nls_member {
     int32_t n;          /* points per axis                                    */

@@ -281,6 +281,167 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
 }
 
 // ---------------------------------------------------------------------------
+// Engine RESIDENT: one workgroup per member holds the member's
+// whole grid on chip (1-D n <= 4096, 2-D n <= 64) and runs all of the call's
+// steps in one launch -- a persistent kernel per step block.  LDS holds the
+// psi state at the start of the step (U) and the psi stage input (Y);
+// registers hold each thread's pumping, reservoir, reservoir stage input and
+// accumulators.  HBM is touched only to load the state
+// at the start and store it at the end.
+// ---------------------------------------------------------------------------
+
+constexpr int RES_MAXP = 4096;
+// threads per workgroup: 1024 in single precision (4 points each); 512 in
+// double precision (8 points each) so the register budget stays at 256
+template <typename T> struct ResShape;
+template <> struct ResShape<float> { static constexpr int THREADS = 1024; };
+template <> struct ResShape<double> { static constexpr int THREADS = 512; };
+
+template <typename T> struct ResidentArgs {
+    const MemberDev<T> *mem;
+    int iters;
+    const T *wtab;
+    cplx_t<T> *psi;
+    T *nres;
+    const T *P;
+};
+
+template <typename T, int ORDER, int BC, int DIM>
+__device__ __forceinline__ cplx_t<T> resident_lap(const cplx_t<T> *Y, int idx, int n, const T *w,
+                                                  const T *wrow) {
+    constexpr int R = (ORDER - 1) / 2;
+    const cplx_t<T> c = Y[idx];
+    if (DIM == 1) {
+        T lr = 0, li = 0;
+        if (BC == BC_PERIODIC) {
+            lr = w[0] * c.x; li = w[0] * c.y;
+#pragma unroll
+            for (int d = 1; d <= R; ++d) {
+                int im = idx - d, ip = idx + d;
+                if (im < 0) im += n;
+                if (ip >= n) ip -= n;
+                lr += w[d] * (Y[im].x + Y[ip].x);
+                li += w[d] * (Y[im].y + Y[ip].y);
+            }
+        } else {
+#pragma unroll
+            for (int d = -R; d <= R; ++d) {
+                const int j = idx + d;
+                if (j < 0 || j >= n) continue;
+                lr += wrow[d + R] * Y[j].x;
+                li += wrow[d + R] * Y[j].y;
+            }
+        }
+        return mk<T>(lr, li);
+    }
+    const int row = idx / n, col = idx - row * n;
+    T lr = (T)2 * w[0] * c.x, li = (T)2 * w[0] * c.y;
+#pragma unroll
+    for (int d = 1; d <= R; ++d) {
+        int cm = col - d, cp = col + d, rm = row - d, rp = row + d;
+        cplx_t<T> a0 = mk<T>(0, 0), a1 = mk<T>(0, 0), b0 = mk<T>(0, 0), b1 = mk<T>(0, 0);
+        if (BC == BC_PERIODIC) {
+            if (cm < 0) cm += n;
+            if (cp >= n) cp -= n;
+            if (rm < 0) rm += n;
+            if (rp >= n) rp -= n;
+            a0 = Y[row * n + cm]; a1 = Y[row * n + cp]; b0 = Y[rm * n + col]; b1 = Y[rp * n + col];
+        } else {
+            if (cm >= 0) a0 = Y[row * n + cm];
+            if (cp < n) a1 = Y[row * n + cp];
+            if (rm >= 0) b0 = Y[rm * n + col];
+            if (rp < n) b1 = Y[rp * n + col];
+        }
+        lr += w[d] * ((a0.x + a1.x) + (b0.x + b1.x));
+        li += w[d] * ((a0.y + a1.y) + (b0.y + b1.y));
+    }
+    return mk<T>(lr, li);
+}
+
+template <typename T, int ORDER, int BC, bool DYN, int DIM>
+__global__ void __launch_bounds__(ResShape<T>::THREADS) k_resident(const ResidentArgs<T> r) {
+    using C = cplx_t<T>;
+    constexpr int RES_THREADS = ResShape<T>::THREADS, RES_PPT = RES_MAXP / RES_THREADS;
+    __shared__ C U[RES_MAXP];
+    __shared__ C Y[RES_MAXP];
+    const MemberDev<T> m = r.mem[blockIdx.x];
+    const int n = m.n;
+    const int N = DIM == 1 ? n : n * n;
+    const int steps = r.iters < m.steps ? r.iters : m.steps;
+    if (N > RES_MAXP || steps <= 0) return;        // the host never plans such members
+    C *psi = r.psi + m.off;
+    T *nres = DYN ? r.nres + m.off : nullptr;
+    const T *P = r.P + m.off;
+    const T *wtab = (DIM == 1 && BC == BC_REF) ? r.wtab + m.tab : nullptr;
+    const T w[4] = {m.w0, m.w1, m.w2, m.w3};
+    const T dt = m.dt;
+    const int tid = threadIdx.x;
+    T pp[RES_PPT], un[RES_PPT], yn[RES_PPT], accn[RES_PPT];
+    C acc[RES_PPT];
+#pragma unroll
+    for (int q = 0; q < RES_PPT; ++q) {
+        const int idx = tid + RES_THREADS * q;
+        pp[q] = 0; un[q] = 0; yn[q] = 0; accn[q] = 0; acc[q] = mk<T>(0, 0);
+        if (idx < N) {
+            const C v = psi[idx];
+            U[idx] = v;
+            Y[idx] = v;
+            pp[q] = P[idx];
+            if (DYN) { un[q] = nres[idx]; yn[q] = un[q]; }
+        }
+    }
+    __syncthreads();
+    for (int it = 0; it < steps; ++it) {
+#pragma unroll
+        for (int s = 1; s <= 4; ++s) {
+            C kk[RES_PPT];
+            T kn[RES_PPT];
+#pragma unroll
+            for (int q = 0; q < RES_PPT; ++q) {
+                const int idx = tid + RES_THREADS * q;
+                kk[q] = mk<T>(0, 0); kn[q] = 0;
+                if (idx < N) {
+                    const C lap = resident_lap<T, ORDER, BC, DIM>(Y, idx, n, w, wtab ? wtab + (size_t)idx * ORDER : nullptr);
+                    rhs_point<T, DYN>(m.c, pp[q], Y[idx], yn[q], lap, kk[q], kn[q]);
+                }
+            }
+            __syncthreads();                         // every read of Y for this stage is done
+            const T wacc = (s == 1 || s == 4) ? (T)1 : (T)2;
+            const T anext = (s == 3) ? (T)1 : (T)0.5;
+#pragma unroll
+            for (int q = 0; q < RES_PPT; ++q) {
+                const int idx = tid + RES_THREADS * q;
+                if (idx >= N) continue;
+                if (s == 1) { acc[q] = kk[q]; accn[q] = kn[q]; }
+                else {
+                    acc[q] = mk<T>(acc[q].x + wacc * kk[q].x, acc[q].y + wacc * kk[q].y);
+                    accn[q] = accn[q] + wacc * kn[q];
+                }
+                const C u = U[idx];
+                if (s < 4) {
+                    Y[idx] = mk<T>(u.x + (kk[q].x * dt) * anext, u.y + (kk[q].y * dt) * anext);
+                    if (DYN) yn[q] = un[q] + (kn[q] * dt) * anext;
+                } else {
+                    const C v = mk<T>(u.x + (acc[q].x * dt) / (T)6, u.y + (acc[q].y * dt) / (T)6);
+                    U[idx] = v;
+                    Y[idx] = v;
+                    if (DYN) { un[q] = un[q] + (accn[q] * dt) / (T)6; yn[q] = un[q]; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RES_PPT; ++q) {
+        const int idx = tid + RES_THREADS * q;
+        if (idx < N) {
+            psi[idx] = U[idx];
+            if (DYN) nres[idx] = un[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 
@@ -307,6 +468,27 @@ static void launch_fused(int order, int bc, bool dyn, const FusedArgs<T> &f, hip
 #define NLS_CASE(O, B, D) \
     if (order == O && bc == B && dyn == D) { \
         hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D>), grid, dim3(256), 0, s, f); return; }
+    NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
+    NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
+    NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
+#undef NLS_CASE
+}
+
+// The double-precision 2-D order-7 variant would spill registers to scratch;
+// it is not built and such batches use the fused or stage engine.
+template <typename T> constexpr bool resident_2d_ok(int order) { return !(sizeof(T) == 8 && order == 7); }
+
+template <typename T>
+static void launch_resident(int dim, int order, int bc, bool dyn, int count, const ResidentArgs<T> &r, hipStream_t s) {
+#define NLS_CASE(O, B, D)                                                                                     \
+    if (order == O && bc == B && dyn == D) {                                                                 \
+        if (dim == 1)                                                                                        \
+            hipLaunchKernelGGL((k_resident<T, O, B, D, 1>), dim3(count), dim3(ResShape<T>::THREADS), 0, s, r); \
+        else if (resident_2d_ok<T>(O))                                                                       \
+            hipLaunchKernelGGL((k_resident<T, O, B, D, (resident_2d_ok<T>(O) ? 2 : 1)>), dim3(count),         \
+                               dim3(ResShape<T>::THREADS), 0, s, r);                                         \
+        return;                                                                                               \
+    }
     NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
     NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
     NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
@@ -381,6 +563,17 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
     const bool dyn = b->dyn;
     C *cur = psi;
     T *curn = nres;
+    if (b->engine == NLS_ENGINE_RESIDENT) {
+        ResidentArgs<T> r;
+        r.mem = (const MemberDev<T> *)b->d_mem;
+        r.iters = iters;
+        r.wtab = (const T *)b->d_wtab;
+        r.psi = psi;
+        r.nres = nres;
+        r.P = P;
+        launch_resident<T>(b->dim, b->order, b->bc, dyn, b->count, r, s);
+        return hip_check(hipGetLastError(), "resident launch");
+    }
     if (b->engine == NLS_ENGINE_FUSED) {
         C *alt = (C *)ws;
         T *altn = dyn ? (T *)(ws + cb) : nullptr;
@@ -446,8 +639,22 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7");
     if (bc != NLS_BC_REF && bc != NLS_BC_PERIODIC) return fail(NLS_EINVAL, "bad bc");
     if (dtype != NLS_F32 && dtype != NLS_F64) return fail(NLS_EINVAL, "bad dtype");
-    if (engine == NLS_ENGINE_AUTO) engine = dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
-    if (engine != NLS_ENGINE_STAGE && engine != NLS_ENGINE_FUSED) return fail(NLS_EINVAL, "bad engine");
+    const int res_max = dim == 1 ? 4096 : 64;
+    const bool res_built = !(dim == 2 && dtype == NLS_F64 && order == 7);
+    if (engine == NLS_ENGINE_AUTO) {
+        bool all_fit = res_built;
+        for (int i = 0; i < count; ++i) all_fit = all_fit && members[i].n <= res_max;
+        engine = all_fit ? NLS_ENGINE_RESIDENT : (dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE);
+    }
+    if (engine != NLS_ENGINE_STAGE && engine != NLS_ENGINE_FUSED && engine != NLS_ENGINE_RESIDENT)
+        return fail(NLS_EINVAL, "bad engine");
+    if (engine == NLS_ENGINE_RESIDENT && !res_built)
+        return fail(NLS_EUNSUPPORTED, "the resident engine has no 2-D order-7 double-precision variant");
+    if (engine == NLS_ENGINE_RESIDENT)
+        for (int i = 0; i < count; ++i)
+            if (members[i].n > res_max)
+                return fail(NLS_EUNSUPPORTED, "member %d: n=%d does not fit one workgroup (resident engine: n <= %d)",
+                            i, members[i].n, res_max);
     if (engine == NLS_ENGINE_FUSED && dim != 2) return fail(NLS_EUNSUPPORTED, "the fused engine is 2-D only");
     const bool dyn = members[0].coeffs[10] != 0.0;
     size_t points = 0;
@@ -536,6 +743,7 @@ size_t nls_batch_workspace_bytes(const nls_batch *b) {
     if (!b) return 0;
     const size_t rs = b->dtype == NLS_F64 ? 8 : 4;
     const size_t cb = align256b(b->points * 2 * rs), rb = align256b(b->points * rs);
+    if (b->engine == NLS_ENGINE_RESIDENT) return 0;
     if (b->engine == NLS_ENGINE_FUSED) return cb + (b->dyn ? rb : 0);
     return 3 * cb + (b->dyn ? 3 * rb : 0);
 }
@@ -560,7 +768,7 @@ int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, v
     if (iters < 0) return fail(NLS_EINVAL, "iters must be >= 0");
     if (!pumping || !psi || (b->dyn && !nres)) return fail(NLS_EINVAL, "NULL device buffer");
     const size_t need = nls_batch_workspace_bytes(b);
-    if (workspace_bytes < need || !workspace)
+    if (workspace_bytes < need || (need && !workspace))
         return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
     if (iters == 0) return NLS_OK;
     hipStream_t s = (hipStream_t)stream;

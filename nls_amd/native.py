@@ -34,7 +34,7 @@ EXPORTS = (
     "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
     "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance", "nls_tune_stage_tile",
 )
-ENGINE = {"auto": -1, "stage": 0, "fused": 1}
+ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2}
 
 
 class NativeError(RuntimeError):

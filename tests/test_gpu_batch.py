@@ -137,3 +137,52 @@ def test_member_alone_equals_member_in_mix(gpu, engine):
         np.testing.assert_array_equal(an, bn)
         alone.close()
     mix.close()
+
+
+@pytest.mark.parametrize("bc", ("periodic", "ref"))
+@pytest.mark.parametrize("dyn", (True, False))
+@pytest.mark.parametrize("order", (3, 5))
+def test_resident_2d_f64_matches_oracle(gpu, oracle, bc, dyn, order):
+    ms = members_2d([16, 40, 64, 33], dyn=dyn)
+    b = Batch(ms, dim=2, order=order, bc=bc, dtype="f64", engine="resident")
+    b.advance(150)
+    for i, m in enumerate(ms):
+        psi, nres = b.member(i)
+        opsi, onres = oracle.rk4(2, m.n, order, 0 if bc == "ref" else 1, m.dt, m.dx, 150, m.P, m.coeffs, m.psi0,
+                                 m.n0)
+        assert rel(psi, opsi) <= TOL64, (i, m.n, rel(psi, opsi))
+        if dyn:
+            assert rel(nres, onres) <= TOL64
+    b.close()
+
+
+@pytest.mark.parametrize("bc", ("ref", "periodic"))
+@pytest.mark.parametrize("order", (3, 5, 7))
+def test_resident_1d_f64_matches_oracle(gpu, oracle, bc, order):
+    ms = members_1d([40, 256, 1000, 4096], ref=(bc == "ref"))
+    b = Batch(ms, dim=1, order=order, bc=bc, dtype="f64", engine="resident")
+    b.advance(200)
+    for i, m in enumerate(ms):
+        psi, nres = b.member(i)
+        opsi, onres = oracle.rk4(1, m.n, order, 0 if bc == "ref" else 1, m.dt, m.dx, 200, m.P, m.coeffs, m.psi0,
+                                 m.n0)
+        assert rel(psi, opsi) <= TOL64, (i, rel(psi, opsi))
+        assert rel(nres, onres) <= TOL64
+
+
+def test_resident_budgets_and_invariance(gpu, oracle):
+    ms = configs.sweep512(count=12, steps=60)
+    small = [m for m in ms if m.n == 64][:4]
+    b = Batch(small, dim=2, order=5, bc="periodic", dtype="f32", engine="resident")
+    b.set_steps([60, 0, 17, 59])
+    b.advance(60)
+    np.testing.assert_array_equal(b.member(1)[0], small[1].psi0.astype(np.complex64))
+    for i in (0, 2, 3):
+        alone = Batch([small[i]], dim=2, order=5, bc="periodic", dtype="f32", engine="resident")
+        alone.advance([60, 0, 17, 59][i])
+        np.testing.assert_array_equal(alone.member(0)[0], b.member(i)[0])
+        alone.close()
+    opsi, _ = oracle.rk4(2, 64, 5, 1, small[2].dt, small[2].dx, 17, small[2].P, small[2].coeffs, small[2].psi0,
+                         small[2].n0)
+    assert rel(b.member(2)[0], opsi) <= TOL32
+    b.close()
