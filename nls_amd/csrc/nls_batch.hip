@@ -137,15 +137,18 @@ template <typename T> struct FusedArgs {
     const T *P;
 };
 
-template <typename T> struct FusedTile;
-template <> struct FusedTile<float> { static constexpr int TY = 32; };
-template <> struct FusedTile<double> { static constexpr int TY = 16; };
+// Output tile height: 32 rows in single and 16 in double precision, halved
+// for order 7 (halo 12) so the per-thread state stays in registers.
+template <typename T, int ORDER> struct FusedTile {
+    static constexpr int TY = (sizeof(T) == 8 ? 16 : 32) / (ORDER == 7 ? 2 : 1);
+};
+static int fused_ty(int dtype, int order) { return (dtype == NLS_F64 ? 16 : 32) / (order == 7 ? 2 : 1); }
 
 template <typename T, int ORDER, int BC, bool DYN>
 __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
     using C = cplx_t<T>;
     constexpr int R = (ORDER - 1) / 2, H = 4 * R;
-    constexpr int TX = 64, TY = FusedTile<T>::TY;
+    constexpr int TX = 64, TY = FusedTile<T, ORDER>::TY;
     constexpr int W = TX + 2 * H, HH = TY + 2 * H, NP = W * HH;
     constexpr int J = (NP + 255) / 256;
     __shared__ C ys[NP];
@@ -184,6 +187,7 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
 
     C uu[J], acc[J], ynext[J];
     T pp[J], nn[J], yn[J], accn[J];
+    int dist[J];                       // distance of the point to the region edge (rings)
     unsigned inside = 0;               // bit j: point j lies on the grid
 
     // load the state, reservoir and pumping of the whole region once
@@ -192,8 +196,12 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
         const int idx = tid + 256 * j;
         uu[j] = mk<T>(0, 0); pp[j] = 0; nn[j] = 0;
         acc[j] = mk<T>(0, 0); ynext[j] = mk<T>(0, 0); accn[j] = 0;
+        dist[j] = -1;
         if (idx < NP) {
             const int ly = idx / W, lx = idx - ly * W;
+            const int dy = ly < HH - 1 - ly ? ly : HH - 1 - ly;
+            const int dx = lx < W - 1 - lx ? lx : W - 1 - lx;
+            dist[j] = dy < dx ? dy : dx;
             int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
             bool ok = true;
             if (BC == BC_PERIODIC) { gx = wrap_index(gx, n); gy = wrap_index(gy, n); }
@@ -211,8 +219,9 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
     }
     __syncthreads();
 
-    // the four stages on the shrinking region: stage s is valid where the
-    // halo still holds s*R valid rings
+    // the four stages on the shrinking region: a point takes part in stage s
+    // while at least s*R rings of halo surround it (the region is stored
+    // row-major with width W, so the LDS index of a point is its region index)
 #pragma unroll
     for (int s = 1; s <= 4; ++s) {
         const int lo = s * R;
@@ -220,10 +229,8 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
         const T anext = (s == 3) ? (T)1 : (T)0.5;  // y_{s+1} = u + (k_s*dt)*anext
 #pragma unroll
         for (int j = 0; j < J; ++j) {
+            if (dist[j] < lo) continue;
             const int idx = tid + 256 * j;
-            if (idx >= NP) break;
-            const int ly = idx / W, lx = idx - ly * W;
-            if (ly < lo || ly >= HH - lo || lx < lo || lx >= W - lo) continue;
             const C y = ys[idx];
             T lr = (T)2 * w[0] * y.x, li = (T)2 * w[0] * y.y;
 #pragma unroll
@@ -245,33 +252,27 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
                 if (DYN) accn[j] = accn[j] + wacc * kn;
             }
             if (s < 4) {
-                const bool in = (inside >> j) & 1u;
-                ynext[j] = in ? mk<T>(uu[j].x + (k.x * dt) * anext, uu[j].y + (k.y * dt) * anext) : mk<T>(0, 0);
+                ynext[j] = mk<T>(uu[j].x + (k.x * dt) * anext, uu[j].y + (k.y * dt) * anext);
+                if (BC != BC_PERIODIC && !((inside >> j) & 1u)) ynext[j] = mk<T>(0, 0);
                 if (DYN) yn[j] = nn[j] + (kn * dt) * anext;
             }
         }
         __syncthreads();
         if (s < 4) {
-            const int lo2 = (s + 1) * R;
+            // stage s+1 reads its input on the stage-s region
 #pragma unroll
-            for (int j = 0; j < J; ++j) {
-                const int idx = tid + 256 * j;
-                if (idx >= NP) break;
-                const int ly = idx / W, lx = idx - ly * W;
-                if (ly < lo2 - R || ly >= HH - lo2 + R || lx < lo2 - R || lx >= W - lo2 + R) continue;
-                ys[idx] = ynext[j];
-            }
+            for (int j = 0; j < J; ++j)
+                if (dist[j] >= lo) ys[tid + 256 * j] = ynext[j];
             __syncthreads();
         }
     }
 
-    // u_new = u + (k1 + 2 k2 + 2 k3 + k4)*dt/6 on the output tile
+    // u_new = u + (k1 + 2 k2 + 2 k3 + k4)*dt/6 on the output tile (dist >= H)
 #pragma unroll
     for (int j = 0; j < J; ++j) {
+        if (dist[j] < H) continue;
         const int idx = tid + 256 * j;
-        if (idx >= NP) break;
         const int ly = idx / W, lx = idx - ly * W;
-        if (ly < H || ly >= H + TY || lx < H || lx >= H + TX) continue;
         const int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
         if (gx >= n || gy >= n) continue;
         const size_t gi = (size_t)gy * n + gx;
@@ -688,7 +689,7 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     // tiles, member by member, row-major inside a member
     std::vector<TileDesc> tiles;
     int tx = 64, ty = BATCH_STAGE_TY;
-    if (engine == NLS_ENGINE_FUSED) ty = dtype == NLS_F64 ? FusedTile<double>::TY : FusedTile<float>::TY;
+    if (engine == NLS_ENGINE_FUSED) ty = fused_ty(dtype, order);
     if (dim == 1) { tx = TILE1; ty = 1; }
     for (int i = 0; i < count; ++i) {
         const int n = members[i].n;

@@ -24,12 +24,14 @@ template <typename T> __device__ __forceinline__ cplx_t<T> mk(T a, T b) {
 template <typename T> struct Coeffs {
     T c3, c4, c5, c6;        // gain, loss, self-interaction, reservoir coupling
     T c11, c12, c13, c14;    // reservoir: d/dt, pumping, decay, depletion
+    T ic11;                  // 1/c11 (double precision, rounded once), 0 when c11 = 0
 };
 
 template <typename T> inline Coeffs<T> make_coeffs(const double *c) {
     Coeffs<T> r;
     r.c3 = (T)c[2]; r.c4 = (T)c[3]; r.c5 = (T)c[4]; r.c6 = (T)c[5];
     r.c11 = (T)c[10]; r.c12 = (T)c[11]; r.c13 = (T)c[12]; r.c14 = (T)c[13];
+    r.ic11 = c[10] != 0.0 ? (T)(1.0 / c[10]) : (T)0;
     return r;
 }
 
@@ -40,6 +42,8 @@ template <typename T> inline Coeffs<T> make_coeffs(const double *c) {
 //   k.re   = (c3 r - c4) y.re + (c5 |y|^2 + c6 r) y.im - (L y).im
 //   k.im   = (c3 r - c4) y.im - (c5 |y|^2 + c6 r) y.re + (L y).re
 //   kn     = (c12 P - (c13 + c14 |y|^2) yn) / c11        (dynamic only)
+// The reservoir's 1/c11 is a precomputed factor (one multiply instead of a
+// division per point and stage; at most one rounding apart).
 template <typename T, bool DYN>
 __device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, T yn, cplx_t<T> lap,
                                           cplx_t<T> &k, T &kn) {
@@ -49,7 +53,7 @@ __device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, 
     const T q = c.c5 * s + c.c6 * r;
     k.x = g * y.x + q * y.y - lap.y;
     k.y = g * y.y - q * y.x + lap.x;
-    if (DYN) kn = (c.c12 * P - (c.c13 + c.c14 * s) * yn) / c.c11;
+    if (DYN) kn = (c.c12 * P - (c.c13 + c.c14 * s) * yn) * c.ic11;
 }
 
 // Periodic index wrap for offsets that may exceed one period.
