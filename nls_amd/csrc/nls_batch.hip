@@ -121,10 +121,16 @@ __global__ void __launch_bounds__(256) k_batch_stage(const BatchArgs<T> b) {
 // ---------------------------------------------------------------------------
 // Engine FUSED (2-D): one RK4 step per launch, halo recomputation.
 // Output tile 64 x TY, halo H = 4R, region W x HH = (64 + 2H) x (TY + 2H).
-// 256 threads; thread t owns region points t + 256 j.  Per owned point the
-// state u, reservoir, pumping, the stage input of the reservoir and the RK4
-// accumulator stay in registers; the stage input of psi sits in LDS so the
-// stencil can read the neighbours.
+// Stage s (1..4) computes the rectangle of points with at least sR halo rings
+// around them.  The region's points are listed in ring order (RingMap): the
+// output tile row-major, then the ring only stages 1..3 compute, then stage
+// 1..2's ring, stage 1's ring and the outermost ring that is only loaded.
+// Every stage's rectangle is then a prefix of the list; with 256 threads
+// owning list entries t + 256 j, a stage's active lanes fill whole
+// wavefronts but for one, and inactive wavefronts skip the stage entirely.
+// Per owned point the state u, reservoir, pumping (as c12 P), the
+// reservoir's stage input and the RK4 accumulators stay in registers; the
+// psi stage input sits in LDS so the stencil can read the neighbours.
 // ---------------------------------------------------------------------------
 
 template <typename T> struct FusedArgs {
@@ -137,21 +143,76 @@ template <typename T> struct FusedArgs {
     const T *P;
 };
 
-// Output tile height: 32 rows in single and 16 in double precision, halved
-// for order 7 (halo 12) so the per-thread state stays in registers.
-template <typename T, int ORDER> struct FusedTile {
-    static constexpr int TY = (sizeof(T) == 8 ? 16 : 32) / (ORDER == 7 ? 2 : 1);
+// Tile shape and launch shape of the fused kernel.  TY: output rows.  NT:
+// threads per workgroup; each thread owns ceil(region / NT) points, so NT
+// trades registers per thread against waves per SIMD (the kernel is bound by
+// latency, and waves per SIMD hide it).  DB: the psi stage input is
+// double-buffered in LDS (stage s reads one buffer and writes the next
+// stage's input into the other: one barrier per stage and no registers
+// holding the next input).  WAVES: waves per SIMD the register budget is
+// sized for.
+template <typename T, int ORDER> struct FusedTile;
+template <int ORDER> struct FusedTile<float, ORDER> {
+    static constexpr int TY = ORDER == 7 ? 16 : 32;
+    static constexpr int NT = 1024;
+    static constexpr bool DB = true;
+    static constexpr int WAVES = 8;
 };
-static int fused_ty(int dtype, int order) { return (dtype == NLS_F64 ? 16 : 32) / (order == 7 ? 2 : 1); }
+template <int ORDER> struct FusedTile<double, ORDER> {
+    static constexpr int TY = ORDER == 7 ? 8 : 16;
+    static constexpr int NT = 512;
+    static constexpr bool DB = false;
+    static constexpr int WAVES = 4;
+};
+// LDS row stride of the region (W + pad), chosen so that the lanes of the
+// ring order's side bands (2R points per row over consecutive rows) fall in
+// distinct banks.
+constexpr int fused_lds_stride(int W, int R) { return W + (R == 1 ? 2 : R == 2 ? 4 : 14); }
+static int fused_ty(int dtype, int order) {
+    return dtype == NLS_F64 ? (order == 7 ? FusedTile<double, 7>::TY : FusedTile<double, 5>::TY)
+                            : (order == 7 ? FusedTile<float, 7>::TY : FusedTile<float, 5>::TY);
+}
+
+// Region coordinates (x | y << 8) in ring order, built at compile time.
+template <int W, int HH, int R> struct RingMap {
+    unsigned short v[W * HH];
+    constexpr RingMap() : v() {
+        int k = 0;
+        for (int s = 4; s >= 0; --s) {           // rectangle s minus rectangle s + 1
+            const int a = s * R, b = (s + 1) * R;
+            for (int y = a; y < HH - a; ++y)
+                for (int x = a; x < W - a; ++x) {
+                    const bool inner = s < 4 && y >= b && y < HH - b && x >= b && x < W - b;
+                    if (!inner) v[k++] = (unsigned short)(x | (y << 8));
+                }
+        }
+    }
+};
+static_assert(RingMap<80, 48, 2>().v[0] == (8 | 8 << 8), "ring order starts at the output tile");
+static_assert(RingMap<80, 48, 2>().v[2048] == (6 | 6 << 8), "stage-3 ring follows the 64 x 32 tile");
+template <int W, int HH, int R> __device__ const RingMap<W, HH, R> ring_map{};
 
 template <typename T, int ORDER, int BC, bool DYN>
-__global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
+__global__ void __launch_bounds__((FusedTile<T, ORDER>::NT), (FusedTile<T, ORDER>::WAVES))
+k_fused_step_2d(const FusedArgs<T> f) {
     using C = cplx_t<T>;
+    using FT = FusedTile<T, ORDER>;
     constexpr int R = (ORDER - 1) / 2, H = 4 * R;
-    constexpr int TX = 64, TY = FusedTile<T, ORDER>::TY;
+    constexpr int TX = 64, TY = FT::TY, NT = FT::NT;
+    constexpr bool DB = FT::DB;
     constexpr int W = TX + 2 * H, HH = TY + 2 * H, NP = W * HH;
-    constexpr int J = (NP + 255) / 256;
-    __shared__ C ys[NP];
+    constexpr int WS = fused_lds_stride(W, R);
+    constexpr int J = (NP + NT - 1) / NT;
+    constexpr int JOUT = TX * TY / NT;    // list entries t + NT j, j < JOUT, are the output tile
+    static_assert(TX * TY % NT == 0 && W < 256 && HH < 256, "fused tile shape");
+    // psi stage input, point (x, y) of the region at ys_mem[b][PAD + y WS + x]:
+    // the front pad makes every stencil read a non-negative constant offset
+    // from the point's own index, so each read is one LDS instruction with an
+    // immediate offset from a single per-point base register
+    constexpr int PAD = R * WS + R;
+    __shared__ C ys_mem[DB ? 2 : 1][HH * WS + 2 * PAD];
+    C *const ya = ys_mem[0] + PAD;            // input of stages 1 and 3 (and 2, 4 unless DB)
+    C *const yb = ys_mem[DB ? 1 : 0] + PAD;   // input of stages 2 and 4
 
     const int t = xcd_tile(f.ntiles, f.chunk);
     if (t < 0) return;
@@ -167,12 +228,10 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
     const int tid = threadIdx.x;
 
     if (f.it >= m.steps) {            // budget used up: pass the state through
-        for (int j = 0; j < J; ++j) {
-            const int idx = tid + 256 * j;
-            if (idx >= NP) break;
-            const int ly = idx / W, lx = idx - ly * W;
-            if (ly < H || ly >= H + TY || lx < H || lx >= H + TX) continue;
-            const int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
+#pragma unroll
+        for (int j = 0; j < JOUT; ++j) {
+            const int k = tid + NT * j;
+            const int gx = td.x0 + (k & (TX - 1)), gy = td.y0 + k / TX;
             if (gx >= n || gy >= n) continue;
             const size_t gi = (size_t)gy * n + gx;
             uo[gi] = u[gi];
@@ -182,102 +241,129 @@ __global__ void __launch_bounds__(256) k_fused_step_2d(const FusedArgs<T> f) {
     }
 
     const T dt = m.dt;
+    const T dt2 = dt * (T)0.5;
+    const T dt6 = dt / (T)6;
     const Coeffs<T> c = m.c;
+    const T w0 = (T)2 * m.w0;          // centre weight of the 2-D Laplacian
     const T w[4] = {m.w0, m.w1, m.w2, m.w3};
+    const unsigned short *map = ring_map<W, HH, R>.v;
 
-    C uu[J], acc[J], ynext[J];
-    T pp[J], nn[J], yn[J], accn[J];
-    int dist[J];                       // distance of the point to the region edge (rings)
-    unsigned inside = 0;               // bit j: point j lies on the grid
+    C uu[J], acc[J], ynext[DB ? 1 : J];
+    T cp[J], nn[J], yn[J], accn[J];
+    int li[J];                         // LDS index of the point
+    unsigned long long inside = 0;     // bit j: point j lies on the grid
+    const bool near = n >= TX + H;     // region offsets stay within one period of the grid
+    static_assert(J <= 64, "inside mask");
 
     // load the state, reservoir and pumping of the whole region once
 #pragma unroll
     for (int j = 0; j < J; ++j) {
-        const int idx = tid + 256 * j;
-        uu[j] = mk<T>(0, 0); pp[j] = 0; nn[j] = 0;
-        acc[j] = mk<T>(0, 0); ynext[j] = mk<T>(0, 0); accn[j] = 0;
-        dist[j] = -1;
-        if (idx < NP) {
-            const int ly = idx / W, lx = idx - ly * W;
-            const int dy = ly < HH - 1 - ly ? ly : HH - 1 - ly;
-            const int dx = lx < W - 1 - lx ? lx : W - 1 - lx;
-            dist[j] = dy < dx ? dy : dx;
+        const int k = tid + NT * j;
+        uu[j] = mk<T>(0, 0); cp[j] = 0; nn[j] = 0;
+        acc[j] = mk<T>(0, 0); accn[j] = 0;
+        if (!DB) ynext[j] = mk<T>(0, 0);
+        li[j] = 0;
+        if (NP % NT == 0 || j < J - 1 || k < NP) {
+            const int e = map[k];
+            const int lx = e & 255, ly = e >> 8;
+            li[j] = ly * WS + lx;
             int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
             bool ok = true;
-            if (BC == BC_PERIODIC) { gx = wrap_index(gx, n); gy = wrap_index(gy, n); }
-            else ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
+            if (BC == BC_PERIODIC) {
+                if (near) { gx = wrap_near(gx, n); gy = wrap_near(gy, n); }
+                else { gx = wrap_index(gx, n); gy = wrap_index(gy, n); }
+            } else {
+                ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
+            }
             if (ok) {
                 const size_t gi = (size_t)gy * n + gx;
-                inside |= 1u << j;
+                inside |= 1ull << j;
                 uu[j] = u[gi];
-                pp[j] = P[gi];
+                cp[j] = c.c12 * P[gi];
                 if (DYN) nn[j] = un[gi];
             }
-            ys[idx] = uu[j];
+            ya[li[j]] = uu[j];
         }
         yn[j] = nn[j];
     }
     __syncthreads();
 
-    // the four stages on the shrinking region: a point takes part in stage s
-    // while at least s*R rings of halo surround it (the region is stored
-    // row-major with width W, so the LDS index of a point is its region index)
+    // Laplacian and right-hand side of the stage input yin at point j
+    auto stage_rhs = [&](const C *yin, int j, C &k, T &kn) {
+        const int idx = li[j];
+        const C y = yin[idx];
+        T lr = w0 * y.x, lm = w0 * y.y;
 #pragma unroll
-    for (int s = 1; s <= 4; ++s) {
-        const int lo = s * R;
-        const T wacc = (s == 1 || s == 4) ? (T)1 : (T)2;
-        const T anext = (s == 3) ? (T)1 : (T)0.5;  // y_{s+1} = u + (k_s*dt)*anext
+        for (int d = 1; d <= R; ++d) {
+            const C a0 = yin[idx - d], a1 = yin[idx + d];
+            const C b0 = yin[idx - d * WS], b1 = yin[idx + d * WS];
+            lr += w[d] * ((a0.x + a1.x) + (b0.x + b1.x));
+            lm += w[d] * ((a0.y + a1.y) + (b0.y + b1.y));
+        }
+        kn = 0;
+        rhs_point_cp<T, DYN>(c, cp[j], y, yn[j], mk<T>(lr, lm), k, kn);
+    };
+
+    // stages 1..3 on the shrinking rectangles.  Stage s computes the points
+    // with at least sR halo rings around them: the first cnt list entries.
+    // The test against cnt is made opaque to the compiler so that each
+    // point's body stays a block of its own; resolved at compile time, the
+    // stage's bodies would merge and all their LDS reads be hoisted together.
+#pragma unroll
+    for (int s = 1; s <= 3; ++s) {
+        const int cnt = (W - 2 * s * R) * (HH - 2 * s * R);
+        int lim = cnt;
+        asm volatile("" : "+s"(lim));
+        const T dta = (s == 3) ? dt : dt2;           // y_{s+1} = u + k_s * dt * (1/2, 1/2, 1)
+        const C *yin = (s == 2) ? yb : ya;
+        C *yout = (s == 2) ? ya : yb;
 #pragma unroll
         for (int j = 0; j < J; ++j) {
-            if (dist[j] < lo) continue;
-            const int idx = tid + 256 * j;
-            const C y = ys[idx];
-            T lr = (T)2 * w[0] * y.x, li = (T)2 * w[0] * y.y;
-#pragma unroll
-            for (int d = 1; d <= R; ++d) {
-                const C a0 = ys[idx - d], a1 = ys[idx + d];
-                const C b0 = ys[idx - d * W], b1 = ys[idx + d * W];
-                lr += w[d] * ((a0.x + a1.x) + (b0.x + b1.x));
-                li += w[d] * ((a0.y + a1.y) + (b0.y + b1.y));
-            }
+            if (NT * j >= cnt) break;
+            if (tid + NT * j >= lim) continue;
             C k;
-            T kn = 0;
-            rhs_point<T, DYN>(c, pp[j], y, yn[j], mk<T>(lr, li), k, kn);
+            T kn;
+            stage_rhs(yin, j, k, kn);
             if (s == 1) {
                 acc[j] = k;
                 if (DYN) accn[j] = kn;
             } else {
-                acc[j].x = acc[j].x + wacc * k.x;
-                acc[j].y = acc[j].y + wacc * k.y;
-                if (DYN) accn[j] = accn[j] + wacc * kn;
+                acc[j].x = acc[j].x + (T)2 * k.x;
+                acc[j].y = acc[j].y + (T)2 * k.y;
+                if (DYN) accn[j] = accn[j] + (T)2 * kn;
             }
-            if (s < 4) {
-                ynext[j] = mk<T>(uu[j].x + (k.x * dt) * anext, uu[j].y + (k.y * dt) * anext);
-                if (BC != BC_PERIODIC && !((inside >> j) & 1u)) ynext[j] = mk<T>(0, 0);
-                if (DYN) yn[j] = nn[j] + (kn * dt) * anext;
-            }
+            C yv = mk<T>(uu[j].x + k.x * dta, uu[j].y + k.y * dta);
+            if (BC != BC_PERIODIC && !((inside >> j) & 1ull)) yv = mk<T>(0, 0);
+            if (DB) yout[li[j]] = yv;
+            else ynext[j] = yv;
+            if (DYN) yn[j] = nn[j] + kn * dta;
         }
         __syncthreads();
-        if (s < 4) {
-            // stage s+1 reads its input on the stage-s region
+        if (!DB) {
+            // stage s+1 reads its input on the stage-s rectangle
 #pragma unroll
-            for (int j = 0; j < J; ++j)
-                if (dist[j] >= lo) ys[tid + 256 * j] = ynext[j];
+            for (int j = 0; j < J; ++j) {
+                if (NT * j >= cnt) break;
+                if (tid + NT * j >= lim) continue;
+                ya[li[j]] = ynext[DB ? 0 : j];
+            }
             __syncthreads();
         }
     }
 
-    // u_new = u + (k1 + 2 k2 + 2 k3 + k4)*dt/6 on the output tile (dist >= H)
+    // stage 4 on the output tile, where it lies on the grid, and
+    // u_new = u + (k1 + 2 k2 + 2 k3 + k4) dt/6
 #pragma unroll
-    for (int j = 0; j < J; ++j) {
-        if (dist[j] < H) continue;
-        const int idx = tid + 256 * j;
-        const int ly = idx / W, lx = idx - ly * W;
-        const int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
+    for (int j = 0; j < JOUT; ++j) {
+        const int k0 = tid + NT * j;
+        const int gx = td.x0 + (k0 & (TX - 1)), gy = td.y0 + k0 / TX;
         if (gx >= n || gy >= n) continue;
+        C k;
+        T kn;
+        stage_rhs(yb, j, k, kn);
         const size_t gi = (size_t)gy * n + gx;
-        uo[gi] = mk<T>(uu[j].x + (acc[j].x * dt) / (T)6, uu[j].y + (acc[j].y * dt) / (T)6);
-        if (DYN) uno[gi] = nn[j] + (accn[j] * dt) / (T)6;
+        uo[gi] = mk<T>(uu[j].x + (acc[j].x + k.x) * dt6, uu[j].y + (acc[j].y + k.y) * dt6);
+        if (DYN) uno[gi] = nn[j] + (accn[j] + kn) * dt6;
     }
 }
 
@@ -468,7 +554,7 @@ static void launch_fused(int order, int bc, bool dyn, const FusedArgs<T> &f, hip
     const dim3 grid(8 * f.chunk);
 #define NLS_CASE(O, B, D) \
     if (order == O && bc == B && dyn == D) { \
-        hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D>), grid, dim3(256), 0, s, f); return; }
+        hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D>), grid, dim3(FusedTile<T, O>::NT), 0, s, f); return; }
     NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
     NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
     NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)

@@ -44,22 +44,37 @@ template <typename T> inline Coeffs<T> make_coeffs(const double *c) {
 //   kn     = (c12 P - (c13 + c14 |y|^2) yn) / c11        (dynamic only)
 // The reservoir's 1/c11 is a precomputed factor (one multiply instead of a
 // division per point and stage; at most one rounding apart).
+// rhs_point_cp takes the product c12 P already formed (kernels that keep a
+// point's pumping in a register across the four stages form it once).
 template <typename T, bool DYN>
-__device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, T yn, cplx_t<T> lap,
-                                          cplx_t<T> &k, T &kn) {
+__device__ __forceinline__ void rhs_point_cp(const Coeffs<T> &c, T cP, cplx_t<T> y, T yn, cplx_t<T> lap,
+                                             cplx_t<T> &k, T &kn) {
     const T s = y.x * y.x + y.y * y.y;
-    const T r = DYN ? yn : c.c12 * P / (c.c13 + c.c14 * s);
+    const T r = DYN ? yn : cP / (c.c13 + c.c14 * s);
     const T g = c.c3 * r - c.c4;
     const T q = c.c5 * s + c.c6 * r;
     k.x = g * y.x + q * y.y - lap.y;
     k.y = g * y.y - q * y.x + lap.x;
-    if (DYN) kn = (c.c12 * P - (c.c13 + c.c14 * s) * yn) * c.ic11;
+    if (DYN) kn = (cP - (c.c13 + c.c14 * s) * yn) * c.ic11;
+}
+
+template <typename T, bool DYN>
+__device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, T yn, cplx_t<T> lap,
+                                          cplx_t<T> &k, T &kn) {
+    rhs_point_cp<T, DYN>(c, c.c12 * P, y, yn, lap, k, kn);
 }
 
 // Periodic index wrap for offsets that may exceed one period.
 __device__ __forceinline__ int wrap_index(int g, int n) {
     while (g < 0) g += n;
     while (g >= n) g -= n;
+    return g;
+}
+
+// Periodic index wrap for -n <= g < 2n.
+__device__ __forceinline__ int wrap_near(int g, int n) {
+    g += g < 0 ? n : 0;
+    g -= g >= n ? n : 0;
     return g;
 }
 
