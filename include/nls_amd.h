@@ -47,6 +47,17 @@ extern "C" {
 #define NLS_F64          1    /* complex128 state, double arithmetic        */
 #define NLS_SCHEME_RK4   0    /* classical RK4 over the FD Laplacian        */
 #define NLS_SCHEME_SSF   1    /* Strang split-step Fourier (periodic only)  */
+/* RK4 engines (nls_problem.engine, nls_batch_create) */
+#define NLS_ENGINE_AUTO  -1   /* nls_problem: FUSED for 2-D, STAGE for 1-D;
+                                 batches: RESIDENT if every member fits, else
+                                 FUSED for 2-D and STAGE for 1-D             */
+#define NLS_ENGINE_STAGE  0   /* one launch per RK4 stage                    */
+#define NLS_ENGINE_FUSED  1   /* one launch per RK4 step, halo recomputed on
+                                 chip (2-D only)                             */
+#define NLS_ENGINE_RESIDENT 2 /* batches only: one launch per advance call, a
+                                 workgroup per member holds its grid on chip
+                                 and runs every step (1-D n <= 4096, 2-D
+                                 n <= 64)                                    */
 
 const char *nls_last_error(void);
 
@@ -96,6 +107,8 @@ typedef struct nls_problem {
     int32_t bc;         /* NLS_BC_REF / NLS_BC_PERIODIC                    */
     int32_t dtype;      /* NLS_F32 / NLS_F64                               */
     int32_t scheme;     /* NLS_SCHEME_RK4 / NLS_SCHEME_SSF                 */
+    int32_t engine;     /* RK4: NLS_ENGINE_AUTO, _STAGE or _FUSED (2-D)    */
+    int32_t reserved;   /* 0                                               */
     double dt, dx;
     double coeffs[23];
 } nls_problem;
@@ -150,15 +163,6 @@ int nls_observables(const nls_problem *p, const void *psi, const void *nres, dou
  * runs one solve per worker process (schedulr/workr.py:39-107); each member
  * follows solve_nls / solve_nls_2d (nls.f90:797, :903).
  * ====================================================================== */
-
-#define NLS_ENGINE_AUTO  -1   /* RESIDENT if every member fits, else FUSED for
-                                 2-D and STAGE for 1-D                       */
-#define NLS_ENGINE_STAGE  0   /* one launch per RK4 stage                    */
-#define NLS_ENGINE_FUSED  1   /* one launch per RK4 step, halo recomputed on
-                                 chip (2-D only)                             */
-#define NLS_ENGINE_RESIDENT 2 /* one launch per advance call: a workgroup per
-                                 member holds its grid on chip and runs every
-                                 step (1-D n <= 4096, 2-D n <= 64)           */
 
 typedef struct nls_member {
     int32_t n;          /* points per axis                                    */

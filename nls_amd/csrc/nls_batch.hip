@@ -128,7 +128,7 @@ __global__ void __launch_bounds__(256) k_batch_stage(const BatchArgs<T> b) {
 // Every stage's rectangle is then a prefix of the list; with 256 threads
 // owning list entries t + 256 j, a stage's active lanes fill whole
 // wavefronts but for one, and inactive wavefronts skip the stage entirely.
-// Per owned point the state u, reservoir, pumping (as c12 P), the
+// Per owned point the state u, reservoir, pumping (as its pump_term), the
 // reservoir's stage input and the RK4 accumulators stay in registers; the
 // psi stage input sits in LDS so the stencil can read the neighbours.
 // ---------------------------------------------------------------------------
@@ -141,6 +141,10 @@ template <typename T> struct FusedArgs {
     const cplx_t<T> *u; const T *un;     // state at the start of the step
     cplx_t<T> *uo; T *uno;               // state at its end
     const T *P;
+    // One grid instead of a member table: grid_ntx > 0 tiles per row of the
+    // grid described by `grid`; tile t sits at (t % grid_ntx, t / grid_ntx).
+    int grid_ntx;
+    MemberDev<T> grid;
 };
 
 // Tile shape and launch shape of the fused kernel.  TY: output rows.  NT:
@@ -159,9 +163,9 @@ template <int ORDER> struct FusedTile<float, ORDER> {
     static constexpr int WAVES = 8;
 };
 template <int ORDER> struct FusedTile<double, ORDER> {
-    static constexpr int TY = ORDER == 7 ? 8 : 16;
-    static constexpr int NT = 512;
-    static constexpr bool DB = false;
+    static constexpr int TY = ORDER == 7 ? 16 : 32;
+    static constexpr int NT = 1024;
+    static constexpr bool DB = true;
     static constexpr int WAVES = 4;
 };
 // LDS row stride of the region (W + pad), chosen so that the lanes of the
@@ -173,9 +177,10 @@ static int fused_ty(int dtype, int order) {
                             : (order == 7 ? FusedTile<float, 7>::TY : FusedTile<float, 5>::TY);
 }
 
-// Region coordinates (x | y << 8) in ring order, built at compile time.
-template <int W, int HH, int R> struct RingMap {
-    unsigned short v[W * HH];
+// Region points in ring order, built at compile time: entry k holds the
+// point's LDS index y WS + x in bits 0..15, x in bits 16..23, y in 24..31.
+template <int W, int HH, int R, int WS> struct RingMap {
+    unsigned v[W * HH];
     constexpr RingMap() : v() {
         int k = 0;
         for (int s = 4; s >= 0; --s) {           // rectangle s minus rectangle s + 1
@@ -183,14 +188,14 @@ template <int W, int HH, int R> struct RingMap {
             for (int y = a; y < HH - a; ++y)
                 for (int x = a; x < W - a; ++x) {
                     const bool inner = s < 4 && y >= b && y < HH - b && x >= b && x < W - b;
-                    if (!inner) v[k++] = (unsigned short)(x | (y << 8));
+                    if (!inner) v[k++] = (unsigned)(y * WS + x) | (unsigned)x << 16 | (unsigned)y << 24;
                 }
         }
     }
 };
-static_assert(RingMap<80, 48, 2>().v[0] == (8 | 8 << 8), "ring order starts at the output tile");
-static_assert(RingMap<80, 48, 2>().v[2048] == (6 | 6 << 8), "stage-3 ring follows the 64 x 32 tile");
-template <int W, int HH, int R> __device__ const RingMap<W, HH, R> ring_map{};
+static_assert(RingMap<80, 48, 2, 84>().v[0] == (8 * 84 + 8 | 8 << 16 | 8 << 24), "ring order starts at the output tile");
+static_assert(RingMap<80, 48, 2, 84>().v[2048] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the 64 x 32 tile");
+template <int W, int HH, int R, int WS> __device__ const RingMap<W, HH, R, WS> ring_map{};
 
 template <typename T, int ORDER, int BC, bool DYN>
 __global__ void __launch_bounds__((FusedTile<T, ORDER>::NT), (FusedTile<T, ORDER>::WAVES))
@@ -216,8 +221,15 @@ k_fused_step_2d(const FusedArgs<T> f) {
 
     const int t = xcd_tile(f.ntiles, f.chunk);
     if (t < 0) return;
-    const TileDesc td = f.tiles[t];
-    const MemberDev<T> m = f.mem[td.m];
+    TileDesc td;
+    MemberDev<T> m;
+    if (f.grid_ntx > 0) {
+        td.m = 0; td.x0 = (t % f.grid_ntx) * TX; td.y0 = (t / f.grid_ntx) * TY;
+        m = f.grid;
+    } else {
+        td = f.tiles[t];
+        m = f.mem[td.m];
+    }
     const int n = m.n;
     const long long off = m.off;
     const C *u = f.u + off;
@@ -246,14 +258,15 @@ k_fused_step_2d(const FusedArgs<T> f) {
     const Coeffs<T> c = m.c;
     const T w0 = (T)2 * m.w0;          // centre weight of the 2-D Laplacian
     const T w[4] = {m.w0, m.w1, m.w2, m.w3};
-    const unsigned short *map = ring_map<W, HH, R>.v;
+    const unsigned *map = ring_map<W, HH, R, WS>.v;
 
     C uu[J], acc[J], ynext[DB ? 1 : J];
     T cp[J], nn[J], yn[J], accn[J];
     int li[J];                         // LDS index of the point
     unsigned long long inside = 0;     // bit j: point j lies on the grid
-    const bool near = n >= TX + H;     // region offsets stay within one period of the grid
     static_assert(J <= 64, "inside mask");
+
+    const bool near = n >= TX + H;     // region offsets stay within one period of the grid
 
     // load the state, reservoir and pumping of the whole region once
 #pragma unroll
@@ -264,10 +277,9 @@ k_fused_step_2d(const FusedArgs<T> f) {
         if (!DB) ynext[j] = mk<T>(0, 0);
         li[j] = 0;
         if (NP % NT == 0 || j < J - 1 || k < NP) {
-            const int e = map[k];
-            const int lx = e & 255, ly = e >> 8;
-            li[j] = ly * WS + lx;
-            int gx = td.x0 + lx - H, gy = td.y0 + ly - H;
+            const unsigned e = map[k];
+            li[j] = e & 0xffff;
+            int gx = td.x0 - H + (int)((e >> 16) & 255), gy = td.y0 - H + (int)(e >> 24);
             bool ok = true;
             if (BC == BC_PERIODIC) {
                 if (near) { gx = wrap_near(gx, n); gy = wrap_near(gy, n); }
@@ -276,10 +288,10 @@ k_fused_step_2d(const FusedArgs<T> f) {
                 ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
             }
             if (ok) {
-                const size_t gi = (size_t)gy * n + gx;
+                const unsigned gi = (unsigned)(gy * n + gx);
                 inside |= 1ull << j;
                 uu[j] = u[gi];
-                cp[j] = c.c12 * P[gi];
+                cp[j] = pump_term<T, DYN>(c, P[gi]);
                 if (DYN) nn[j] = un[gi];
             }
             ya[li[j]] = uu[j];
@@ -596,6 +608,62 @@ __global__ void k_batch_radial_tables(const MemberDev<T> *mem, int count, int or
 
 static size_t align256b(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// Plain 2-D RK4 solve of one n x n grid on the fused kernel.  The member
+// descriptor and the tile grid travel in the kernel arguments; the workspace
+// holds the other half of the state ping-pong.
+size_t fused_grid_workspace_bytes(const nls_problem *p) {
+    const size_t N = (size_t)p->n * p->n, rs = p->dtype == NLS_F64 ? 8 : 4;
+    return align256b(N * 2 * rs) + (p->coeffs[10] != 0.0 ? align256b(N * rs) : 0);
+}
+
+template <typename T>
+static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws,
+                          hipStream_t s) {
+    using C = cplx_t<T>;
+    const size_t N = (size_t)p->n * p->n;
+    const bool dyn = p->coeffs[10] != 0.0;
+    const int ty = fused_ty(p->dtype, p->order);
+    FusedArgs<T> f;
+    memset(&f, 0, sizeof f);
+    f.grid_ntx = (p->n + 63) / 64;
+    f.ntiles = f.grid_ntx * ((p->n + ty - 1) / ty);
+    f.chunk = (f.ntiles + 7) / 8;
+    MemberDev<T> &d = f.grid;
+    d.n = p->n;
+    d.steps = 0x7fffffff;
+    d.tab = -1;
+    d.dt = (T)p->dt;
+    d.c = make_coeffs<T>(p->coeffs);
+    T w[4] = {0, 0, 0, 0};
+    for (int q = 0; q <= (p->order - 1) / 2; ++q) w[q] = (T)stencil_s2(p->order, q, p->dx);
+    d.w0 = w[0]; d.w1 = w[1]; d.w2 = w[2]; d.w3 = w[3];
+    f.P = P;
+    C *cur = psi, *alt = (C *)ws;
+    T *curn = nres, *altn = dyn ? (T *)(ws + align256b(N * sizeof(C))) : nullptr;
+    for (int it = 0; it < iters; ++it) {
+        f.it = it;
+        f.u = cur; f.un = curn; f.uo = alt; f.uno = altn;
+        launch_fused<T>(p->order, p->bc, dyn, f, s);
+        std::swap(cur, alt);
+        std::swap(curn, altn);
+    }
+    int rc = hip_check(hipGetLastError(), "fused step launch");
+    if (rc) return rc;
+    if (cur != psi) {
+        rc = hip_check(hipMemcpyAsync(psi, cur, N * sizeof(C), hipMemcpyDeviceToDevice, s), "copy state");
+        if (rc) return rc;
+        if (dyn) rc = hip_check(hipMemcpyAsync(nres, curn, N * sizeof(T), hipMemcpyDeviceToDevice, s), "copy reservoir");
+    }
+    return rc;
+}
+
+int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
+                       hipStream_t s) {
+    if (p->dtype == NLS_F64)
+        return fused_grid_run<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)ws, s);
+    return fused_grid_run<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);
+}
+
 }  // namespace nls
 
 using namespace nls;
@@ -665,6 +733,7 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
         C *alt = (C *)ws;
         T *altn = dyn ? (T *)(ws + cb) : nullptr;
         FusedArgs<T> f;
+        memset(&f, 0, sizeof f);
         f.mem = (const MemberDev<T> *)b->d_mem;
         f.tiles = (const TileDesc *)b->d_tiles;
         f.ntiles = b->ntiles;

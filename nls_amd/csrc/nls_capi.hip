@@ -43,7 +43,17 @@ int check_problem(const nls_problem *p) {
     if (p->scheme == NLS_SCHEME_SSF && p->bc != NLS_BC_PERIODIC)
         return fail(NLS_EUNSUPPORTED, "the split-step Fourier scheme needs periodic boundaries");
     if (p->scheme == NLS_SCHEME_SSF) return ssf_check(p);
+    if (p->engine != NLS_ENGINE_AUTO && p->engine != NLS_ENGINE_STAGE && p->engine != NLS_ENGINE_FUSED)
+        return fail(NLS_EINVAL, "engine must be NLS_ENGINE_AUTO, _STAGE or _FUSED (got %d)", p->engine);
+    if (p->engine == NLS_ENGINE_FUSED && p->dim != 2)
+        return fail(NLS_EUNSUPPORTED, "the fused engine is 2-D only");
     return NLS_OK;
+}
+
+// RK4 engine of a plain solve: the fused one-launch-per-step kernel for 2-D
+// grids, the per-stage kernels for 1-D lines (and when asked for).
+bool use_fused(const nls_problem *p) {
+    return p->scheme == NLS_SCHEME_RK4 && p->dim == 2 && p->engine != NLS_ENGINE_STAGE;
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -169,6 +179,7 @@ int nls_version(int *major, int *minor, int *patch) {
 size_t nls_workspace_bytes(const nls_problem *p) {
     if (check_problem(p)) return 0;
     if (p->scheme == NLS_SCHEME_SSF) return ssf_workspace_bytes(p);
+    if (use_fused(p)) return fused_grid_workspace_bytes(p);
     return rk4_workspace(p, nullptr);
 }
 
@@ -185,6 +196,7 @@ int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi,
     if (iters == 0) return NLS_OK;
     hipStream_t s = (hipStream_t)stream;
     if (p->scheme == NLS_SCHEME_SSF) return ssf_advance(p, iters, pumping, psi, nres, workspace, s);
+    if (use_fused(p)) return fused_grid_advance(p, iters, pumping, psi, nres, workspace, s);
     if (p->dtype == NLS_F64)
         return rk4_advance<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres,
                                    (char *)workspace, s);
@@ -249,6 +261,7 @@ static int dropin_solve(int dim, float dt, float dx, int n, int order, int iters
     nls_problem p;
     memset(&p, 0, sizeof p);
     p.dim = dim; p.n = n; p.order = order; p.bc = NLS_BC_REF; p.dtype = NLS_F32; p.scheme = NLS_SCHEME_RK4;
+    p.engine = NLS_ENGINE_AUTO;
     p.dt = dt; p.dx = dx;
     for (int k = 0; k < 23; ++k) p.coeffs[k] = coeffs[k];
     int rc = check_problem(&p);

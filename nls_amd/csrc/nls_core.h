@@ -24,14 +24,17 @@ template <typename T> __device__ __forceinline__ cplx_t<T> mk(T a, T b) {
 template <typename T> struct Coeffs {
     T c3, c4, c5, c6;        // gain, loss, self-interaction, reservoir coupling
     T c11, c12, c13, c14;    // reservoir: d/dt, pumping, decay, depletion
-    T ic11;                  // 1/c11 (double precision, rounded once), 0 when c11 = 0
+    T d12, d13, d14;         // c12/c11, c13/c11, c14/c11 (formed in double, rounded once); 0 when c11 = 0
 };
 
 template <typename T> inline Coeffs<T> make_coeffs(const double *c) {
     Coeffs<T> r;
     r.c3 = (T)c[2]; r.c4 = (T)c[3]; r.c5 = (T)c[4]; r.c6 = (T)c[5];
     r.c11 = (T)c[10]; r.c12 = (T)c[11]; r.c13 = (T)c[12]; r.c14 = (T)c[13];
-    r.ic11 = c[10] != 0.0 ? (T)(1.0 / c[10]) : (T)0;
+    const bool dyn = c[10] != 0.0;
+    r.d12 = dyn ? (T)(c[11] / c[10]) : (T)0;
+    r.d13 = dyn ? (T)(c[12] / c[10]) : (T)0;
+    r.d14 = dyn ? (T)(c[13] / c[10]) : (T)0;
     return r;
 }
 
@@ -42,10 +45,17 @@ template <typename T> inline Coeffs<T> make_coeffs(const double *c) {
 //   k.re   = (c3 r - c4) y.re + (c5 |y|^2 + c6 r) y.im - (L y).im
 //   k.im   = (c3 r - c4) y.im - (c5 |y|^2 + c6 r) y.re + (L y).re
 //   kn     = (c12 P - (c13 + c14 |y|^2) yn) / c11        (dynamic only)
-// The reservoir's 1/c11 is a precomputed factor (one multiply instead of a
-// division per point and stage; at most one rounding apart).
-// rhs_point_cp takes the product c12 P already formed (kernels that keep a
-// point's pumping in a register across the four stages form it once).
+// The reservoir's 1/c11 is folded into its coefficients once
+// (kn = d12 P - (d13 + d14 |y|^2) yn), and each psi component is two fused
+// multiply-adds; both regroupings move results by rounding only.
+//
+// rhs_point_cp takes the pumping term already formed as cP = c12 P
+// (adiabatic) or d12 P (dynamic), so kernels that keep a point's pumping in
+// a register across the four stages form it once (pump_term).
+template <typename T, bool DYN> __device__ __forceinline__ T pump_term(const Coeffs<T> &c, T P) {
+    return DYN ? c.d12 * P : c.c12 * P;
+}
+
 template <typename T, bool DYN>
 __device__ __forceinline__ void rhs_point_cp(const Coeffs<T> &c, T cP, cplx_t<T> y, T yn, cplx_t<T> lap,
                                              cplx_t<T> &k, T &kn) {
@@ -53,15 +63,15 @@ __device__ __forceinline__ void rhs_point_cp(const Coeffs<T> &c, T cP, cplx_t<T>
     const T r = DYN ? yn : cP / (c.c13 + c.c14 * s);
     const T g = c.c3 * r - c.c4;
     const T q = c.c5 * s + c.c6 * r;
-    k.x = g * y.x + q * y.y - lap.y;
-    k.y = g * y.y - q * y.x + lap.x;
-    if (DYN) kn = (cP - (c.c13 + c.c14 * s) * yn) * c.ic11;
+    k.x = fma(g, y.x, fma(q, y.y, -lap.y));
+    k.y = fma(g, y.y, fma(-q, y.x, lap.x));
+    if (DYN) kn = fma(-fma(c.d14, s, c.d13), yn, cP);
 }
 
 template <typename T, bool DYN>
 __device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, T yn, cplx_t<T> lap,
                                           cplx_t<T> &k, T &kn) {
-    rhs_point_cp<T, DYN>(c, c.c12 * P, y, yn, lap, k, kn);
+    rhs_point_cp<T, DYN>(c, pump_term<T, DYN>(c, P), y, yn, lap, k, kn);
 }
 
 // Periodic index wrap for offsets that may exceed one period.

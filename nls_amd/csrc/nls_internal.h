@@ -12,6 +12,13 @@ int hip_check(hipError_t e, const char *what);     // NLS_OK or NLS_EHIP/NLS_ENO
 int check_problem(const nls_problem *p);
 size_t points_of(const nls_problem *p);
 
+bool use_fused(const nls_problem *p);              // plain RK4 solve on the fused engine
+
+// plain 2-D RK4 solve of one grid on the fused kernel (nls_batch.hip)
+size_t fused_grid_workspace_bytes(const nls_problem *p);
+int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
+                       hipStream_t s);
+
 // split-step Fourier scheme (nls_ssf.hip)
 size_t ssf_workspace_bytes(const nls_problem *p);
 int ssf_check(const nls_problem *p);               // size limits of the scheme

@@ -210,6 +210,7 @@ static int dropin_mu(int dim, float dx, int n, const float *pumping, const float
     nls_problem p;
     memset(&p, 0, sizeof p);
     p.dim = dim; p.n = n; p.order = 5; p.bc = NLS_BC_REF; p.dtype = NLS_F32; p.scheme = NLS_SCHEME_RK4;
+    p.engine = NLS_ENGINE_AUTO;
     p.dt = 0.0; p.dx = dx;
     for (int k = 0; k < 23; ++k) p.coeffs[k] = coeffs[k];
     p.coeffs[10] = 0.0;                    // the reference's hamiltonian is adiabatic

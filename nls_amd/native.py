@@ -44,16 +44,21 @@ class NativeError(RuntimeError):
 class ProblemDesc(C.Structure):
     """Mirror of ``nls_problem`` (include/nls_amd.h)."""
     _fields_ = [("dim", C.c_int32), ("n", C.c_int32), ("order", C.c_int32), ("bc", C.c_int32),
-                ("dtype", C.c_int32), ("scheme", C.c_int32), ("dt", C.c_double), ("dx", C.c_double),
-                ("coeffs", C.c_double * 23)]
+                ("dtype", C.c_int32), ("scheme", C.c_int32), ("engine", C.c_int32), ("reserved", C.c_int32),
+                ("dt", C.c_double), ("dx", C.c_double), ("coeffs", C.c_double * 23)]
 
     @classmethod
-    def make(cls, dim, n, order, dt, dx, coeffs, bc="ref", dtype="f32", scheme="rk4"):
+    def make(cls, dim, n, order, dt, dx, coeffs, bc="ref", dtype="f32", scheme="rk4", engine="auto"):
+        """engine: RK4 engine of a plain solve -- "auto" (fused for 2-D,
+        stage for 1-D), "stage" (one launch per RK4 stage) or "fused"
+        (one launch per step, 2-D only)."""
         p = cls()
         p.dim, p.n, p.order = int(dim), int(n), int(order)
         p.bc = BC[bc] if isinstance(bc, str) else int(bc)
         p.dtype = DTYPE[dtype] if isinstance(dtype, str) else int(dtype)
         p.scheme = SCHEME[scheme] if isinstance(scheme, str) else int(scheme)
+        p.engine = ENGINE[engine] if isinstance(engine, str) else int(engine)
+        p.reserved = 0
         p.dt, p.dx = float(dt), float(dx)
         c = np.zeros(23)
         cc = np.asarray(coeffs, np.float64).ravel()

@@ -41,11 +41,15 @@ def small_case(dim, n, bc, dyn, seed=7):
     return P, psi0, n0, c
 
 
-CASES = list(itertools.product((1, 2), ("ref", "periodic"), (False, True), (3, 5, 7)))
+# 1-D lines run on the per-stage kernels; 2-D grids on both RK4 engines
+# (the fused one-launch-per-step kernel is the default for 2-D)
+CASES = [(dim, bc, dyn, order, engine)
+         for dim, bc, dyn, order in itertools.product((1, 2), ("ref", "periodic"), (False, True), (3, 5, 7))
+         for engine in (("stage",) if dim == 1 else ("stage", "fused"))]
 
 
-@pytest.mark.parametrize("dim,bc,dyn,order", CASES)
-def test_rk4_f64_matches_oracle(gpu, oracle, dim, bc, dyn, order):
+@pytest.mark.parametrize("dim,bc,dyn,order,engine", CASES)
+def test_rk4_f64_matches_oracle(gpu, oracle, dim, bc, dyn, order, engine):
     n = 96 if dim == 1 else 40
     dx = 0.1 if dim == 1 else 0.2
     P, psi0, n0, c = small_case(dim, n, bc, dyn)
@@ -53,7 +57,7 @@ def test_rk4_f64_matches_oracle(gpu, oracle, dim, bc, dyn, order):
     dt = 2e-3 if dim == 1 else 1e-3
     if dim == 1 and bc == "ref":
         dt = 5e-4          # the radial operator's 1/r rows are stiffer
-    desc = native.ProblemDesc.make(dim, n, order, dt, dx, c, bc=bc, dtype="f64")
+    desc = native.ProblemDesc.make(dim, n, order, dt, dx, c, bc=bc, dtype="f64", engine=engine)
     psi, nres = native.solve_host(desc, steps, P, psi0, n0)
     opsi, onres = oracle.rk4(dim, n, order, 0 if bc == "ref" else 1, dt, dx, steps, P, c, psi0, n0)
     assert np.all(np.isfinite(psi))
@@ -62,14 +66,14 @@ def test_rk4_f64_matches_oracle(gpu, oracle, dim, bc, dyn, order):
         assert rel(nres, onres) <= TOL64
 
 
-@pytest.mark.parametrize("dim,bc,dyn,order", CASES)
-def test_rk4_f32_within_tolerance_of_f64_oracle(gpu, oracle, dim, bc, dyn, order):
+@pytest.mark.parametrize("dim,bc,dyn,order,engine", CASES)
+def test_rk4_f32_within_tolerance_of_f64_oracle(gpu, oracle, dim, bc, dyn, order, engine):
     n = 96 if dim == 1 else 40
     dx = 0.1 if dim == 1 else 0.2
     P, psi0, n0, c = small_case(dim, n, bc, dyn)
     steps = 300
     dt = 5e-4 if (dim == 1 and bc == "ref") else (2e-3 if dim == 1 else 1e-3)
-    desc = native.ProblemDesc.make(dim, n, order, dt, dx, c, bc=bc, dtype="f32")
+    desc = native.ProblemDesc.make(dim, n, order, dt, dx, c, bc=bc, dtype="f32", engine=engine)
     psi, nres = native.solve_host(desc, steps, P, psi0, n0)
     opsi, onres = oracle.rk4(dim, n, order, 0 if bc == "ref" else 1, dt, dx, steps, P, c, psi0, n0)
     assert rel(psi, opsi) <= TOL32, rel(psi, opsi)
