@@ -47,17 +47,20 @@ extern "C" {
 #define NLS_F64          1    /* complex128 state, double arithmetic        */
 #define NLS_SCHEME_RK4   0    /* classical RK4 over the FD Laplacian        */
 #define NLS_SCHEME_SSF   1    /* Strang split-step Fourier (periodic only)  */
+/* nls_problem.flags */
+#define NLS_FLAG_EAGER    1   /* issue every launch directly: no step-block
+                                 graphs (nls_advance)                        */
 /* RK4 engines (nls_problem.engine, nls_batch_create) */
-#define NLS_ENGINE_AUTO  -1   /* nls_problem: FUSED for 2-D, STAGE for 1-D;
-                                 batches: RESIDENT if every member fits, else
-                                 FUSED for 2-D and STAGE for 1-D             */
+#define NLS_ENGINE_AUTO  -1   /* RESIDENT if the instance (every member of a
+                                 batch) fits, else FUSED for 2-D and STAGE
+                                 for 1-D                                     */
 #define NLS_ENGINE_STAGE  0   /* one launch per RK4 stage                    */
 #define NLS_ENGINE_FUSED  1   /* one launch per RK4 step, halo recomputed on
                                  chip (2-D only)                             */
-#define NLS_ENGINE_RESIDENT 2 /* batches only: one launch per advance call, a
-                                 workgroup per member holds its grid on chip
-                                 and runs every step (1-D n <= 4096, 2-D
-                                 n <= 64)                                    */
+#define NLS_ENGINE_RESIDENT 2 /* a workgroup per instance holds its grid on
+                                 chip and runs a block of steps per launch
+                                 (1-D n <= 4096, 2-D n <= 64; not FP64 2-D
+                                 order 7)                                    */
 
 const char *nls_last_error(void);
 
@@ -107,8 +110,8 @@ typedef struct nls_problem {
     int32_t bc;         /* NLS_BC_REF / NLS_BC_PERIODIC                    */
     int32_t dtype;      /* NLS_F32 / NLS_F64                               */
     int32_t scheme;     /* NLS_SCHEME_RK4 / NLS_SCHEME_SSF                 */
-    int32_t engine;     /* RK4: NLS_ENGINE_AUTO, _STAGE or _FUSED (2-D)    */
-    int32_t reserved;   /* 0                                               */
+    int32_t engine;     /* RK4: NLS_ENGINE_* (AUTO picks by size)          */
+    int32_t flags;      /* NLS_FLAG_* bits, 0 by default                   */
     double dt, dx;
     double coeffs[23];
 } nls_problem;
@@ -124,9 +127,18 @@ size_t nls_workspace_bytes(const nls_problem *p);
 /* Device-memory solve on a stream: psi, nres (dynamic reservoir only) and
  * pumping are device arrays; psi/nres are advanced in place.  workspace is a
  * device buffer of at least nls_workspace_bytes(p).  Enqueues work and
- * returns; the caller synchronises the stream. */
+ * returns; the caller synchronises the stream.
+ * Solves of at least 128 steps on the fused or per-stage engine replay a
+ * captured block of 64 steps as one hipGraph launch (unless
+ * NLS_FLAG_EAGER): the graph is captured on a private stream of the calling
+ * thread, cached per thread by device, problem and buffer addresses (up to 8
+ * entries), and launched on `stream`. */
 int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres,
                 void *workspace, size_t workspace_bytes, void *stream);
+
+/* Frees the calling thread's cached step-block graphs (after synchronising
+ * the device).  Optional: the cache is bounded. */
+int nls_release(void);
 
 /* One evaluation of the right-hand side (hamiltonian / hamiltonian_2d,
  * nls.f90:621-650, :841-870) on device arrays: dpsi = F_psi, dn = F_n

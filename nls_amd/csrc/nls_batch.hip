@@ -616,6 +616,21 @@ size_t fused_grid_workspace_bytes(const nls_problem *p) {
     return align256b(N * 2 * rs) + (p->coeffs[10] != 0.0 ? align256b(N * rs) : 0);
 }
 
+// Member descriptor of a plain (single-instance) solve.
+template <typename T> static MemberDev<T> plain_member(const nls_problem *p) {
+    MemberDev<T> d;
+    memset(&d, 0, sizeof d);
+    d.n = p->n;
+    d.steps = 0x7fffffff;
+    d.tab = -1;
+    d.dt = (T)p->dt;
+    d.c = make_coeffs<T>(p->coeffs);
+    T w[4] = {0, 0, 0, 0};
+    for (int q = 0; q <= (p->order - 1) / 2; ++q) w[q] = (T)stencil_s2(p->order, q, p->dx);
+    d.w0 = w[0]; d.w1 = w[1]; d.w2 = w[2]; d.w3 = w[3];
+    return d;
+}
+
 template <typename T>
 static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws,
                           hipStream_t s) {
@@ -628,15 +643,7 @@ static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T>
     f.grid_ntx = (p->n + 63) / 64;
     f.ntiles = f.grid_ntx * ((p->n + ty - 1) / ty);
     f.chunk = (f.ntiles + 7) / 8;
-    MemberDev<T> &d = f.grid;
-    d.n = p->n;
-    d.steps = 0x7fffffff;
-    d.tab = -1;
-    d.dt = (T)p->dt;
-    d.c = make_coeffs<T>(p->coeffs);
-    T w[4] = {0, 0, 0, 0};
-    for (int q = 0; q <= (p->order - 1) / 2; ++q) w[q] = (T)stencil_s2(p->order, q, p->dx);
-    d.w0 = w[0]; d.w1 = w[1]; d.w2 = w[2]; d.w3 = w[3];
+    f.grid = plain_member<T>(p);
     f.P = P;
     C *cur = psi, *alt = (C *)ws;
     T *curn = nres, *altn = dyn ? (T *)(ws + align256b(N * sizeof(C))) : nullptr;
@@ -655,6 +662,59 @@ static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T>
         if (dyn) rc = hip_check(hipMemcpyAsync(nres, curn, N * sizeof(T), hipMemcpyDeviceToDevice, s), "copy reservoir");
     }
     return rc;
+}
+
+// Plain solve of one line (n <= 4096) or small grid (n <= 64) on the
+// resident kernel: one workgroup holds the instance on chip and runs a block
+// of steps per launch.  Workspace: the radial row table of a 1-D ref solve.
+bool resident_fits(const nls_problem *p) {
+    if (p->scheme != NLS_SCHEME_RK4) return false;
+    if (p->dim == 1) return p->n <= RES_MAXP;
+    return (size_t)p->n * p->n <= (size_t)RES_MAXP && !(p->dtype == NLS_F64 && p->order == 7);
+}
+
+size_t resident_workspace_bytes(const nls_problem *p) {
+    const size_t rs = p->dtype == NLS_F64 ? 8 : 4;
+    return 256 + (p->dim == 1 && p->bc == NLS_BC_REF ? align256b((size_t)p->n * p->order * rs) : 0);
+}
+
+// Writes a by-value member descriptor into device memory (the one-entry
+// member table of a plain resident solve), so no host-to-device copy is needed.
+template <typename T> __global__ void k_put_member(MemberDev<T> *dst, const MemberDev<T> v) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) *dst = v;
+}
+
+template <typename T>
+static int resident_run(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws,
+                        hipStream_t s) {
+    static_assert(sizeof(MemberDev<T>) <= 256, "member descriptor fits its workspace slot");
+    MemberDev<T> one = plain_member<T>(p);
+    ResidentArgs<T> r;
+    memset(&r, 0, sizeof r);
+    r.psi = psi;
+    r.nres = nres;
+    r.P = P;
+    if (p->dim == 1 && p->bc == NLS_BC_REF) {
+        launch_radial_table<T>(p->n, p->order, p->dx, (T *)(ws + 256), s);
+        r.wtab = (const T *)(ws + 256);
+        one.tab = 0;
+    }
+    hipLaunchKernelGGL((k_put_member<T>), dim3(1), dim3(64), 0, s, (MemberDev<T> *)ws, one);
+    r.mem = (const MemberDev<T> *)ws;
+    // bounded launches: each runs at most 4096 steps
+    for (int done = 0; done < iters;) {
+        r.iters = iters - done < 4096 ? iters - done : 4096;
+        launch_resident<T>(p->dim, p->order, p->bc, p->coeffs[10] != 0.0, 1, r, s);
+        done += r.iters;
+    }
+    return hip_check(hipGetLastError(), "resident launch");
+}
+
+int resident_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
+                     hipStream_t s) {
+    if (p->dtype == NLS_F64)
+        return resident_run<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)ws, s);
+    return resident_run<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);
 }
 
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
@@ -720,6 +780,7 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
     T *curn = nres;
     if (b->engine == NLS_ENGINE_RESIDENT) {
         ResidentArgs<T> r;
+        memset(&r, 0, sizeof r);
         r.mem = (const MemberDev<T> *)b->d_mem;
         r.iters = iters;
         r.wtab = (const T *)b->d_wtab;

@@ -43,17 +43,26 @@ int check_problem(const nls_problem *p) {
     if (p->scheme == NLS_SCHEME_SSF && p->bc != NLS_BC_PERIODIC)
         return fail(NLS_EUNSUPPORTED, "the split-step Fourier scheme needs periodic boundaries");
     if (p->scheme == NLS_SCHEME_SSF) return ssf_check(p);
-    if (p->engine != NLS_ENGINE_AUTO && p->engine != NLS_ENGINE_STAGE && p->engine != NLS_ENGINE_FUSED)
-        return fail(NLS_EINVAL, "engine must be NLS_ENGINE_AUTO, _STAGE or _FUSED (got %d)", p->engine);
+    if (p->engine < NLS_ENGINE_AUTO || p->engine > NLS_ENGINE_RESIDENT)
+        return fail(NLS_EINVAL, "engine must be one of NLS_ENGINE_* (got %d)", p->engine);
     if (p->engine == NLS_ENGINE_FUSED && p->dim != 2)
         return fail(NLS_EUNSUPPORTED, "the fused engine is 2-D only");
+    if (p->engine == NLS_ENGINE_RESIDENT && p->scheme == NLS_SCHEME_RK4 && !resident_fits(p))
+        return fail(NLS_EUNSUPPORTED, "the resident engine holds 1-D n <= 4096 or 2-D n <= 64 "
+                                      "(not double-precision 2-D order 7)");
     return NLS_OK;
 }
 
-// RK4 engine of a plain solve: the fused one-launch-per-step kernel for 2-D
-// grids, the per-stage kernels for 1-D lines (and when asked for).
-bool use_fused(const nls_problem *p) {
-    return p->scheme == NLS_SCHEME_RK4 && p->dim == 2 && p->engine != NLS_ENGINE_STAGE;
+// RK4 engine of a plain solve.  AUTO: the resident kernel when the instance
+// fits on chip (one launch runs a block of steps) -- every 1-D line up to
+// 4096 points, 2-D grids up to 64 x 64 in single precision (in double
+// precision one workgroup computing the whole grid is slower than the fused
+// kernel's tiles) -- else the fused one-launch-per-step kernel for 2-D grids
+// and the per-stage kernels for 1-D lines.
+int plain_engine(const nls_problem *p) {
+    if (p->engine != NLS_ENGINE_AUTO) return p->engine;
+    if (resident_fits(p) && (p->dim == 1 || p->dtype == NLS_F32)) return NLS_ENGINE_RESIDENT;
+    return p->dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -179,8 +188,123 @@ int nls_version(int *major, int *minor, int *patch) {
 size_t nls_workspace_bytes(const nls_problem *p) {
     if (check_problem(p)) return 0;
     if (p->scheme == NLS_SCHEME_SSF) return ssf_workspace_bytes(p);
-    if (use_fused(p)) return fused_grid_workspace_bytes(p);
-    return rk4_workspace(p, nullptr);
+    switch (plain_engine(p)) {
+    case NLS_ENGINE_FUSED: return fused_grid_workspace_bytes(p);
+    case NLS_ENGINE_RESIDENT: return resident_workspace_bytes(p);
+    default: return rk4_workspace(p, nullptr);
+    }
+}
+
+// Enqueues iters steps as plain launches.
+static int advance_direct(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
+                          hipStream_t s) {
+    if (p->scheme == NLS_SCHEME_SSF) return ssf_advance(p, iters, pumping, psi, nres, ws, s);
+    const int engine = plain_engine(p);
+    if (engine == NLS_ENGINE_FUSED) return fused_grid_advance(p, iters, pumping, psi, nres, ws, s);
+    if (engine == NLS_ENGINE_RESIDENT) return resident_advance(p, iters, pumping, psi, nres, ws, s);
+    if (p->dtype == NLS_F64)
+        return rk4_advance<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)ws, s);
+    return rk4_advance<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);
+}
+
+// ---- step-block graphs -------------------------------------------------------
+// A block of GRAPH_BLOCK steps (even, so the state ping-pong ends where it
+// started and the block holds kernel nodes only) is captured once on a
+// private stream of the calling thread and replayed as one graph launch.
+// Entries are keyed by device, problem and buffer addresses; the cache holds
+// GRAPH_CACHE entries per thread (least recently used evicted).  Capture
+// streams have the default priority.
+constexpr int GRAPH_BLOCK = 64;
+constexpr size_t GRAPH_CACHE = 8;
+
+struct GraphEntry {
+    int device;
+    nls_problem p;
+    const void *P;
+    void *psi, *nres, *ws;
+    hipGraphExec_t exec;
+    unsigned long long used;
+};
+
+struct GraphCache {
+    std::vector<GraphEntry> entries;
+    std::vector<hipStream_t> capture;   // per device, created on first use
+    unsigned long long clock = 0;
+};
+
+// Never destroyed at thread exit (the HIP runtime may be gone by then);
+// nls_release() frees it explicitly.
+static thread_local GraphCache *g_graphs = nullptr;
+
+static bool graphs_apply(const nls_problem *p, int iters) {
+    if ((p->flags & NLS_FLAG_EAGER) || iters < 2 * GRAPH_BLOCK || p->scheme != NLS_SCHEME_RK4) return false;
+    const int engine = plain_engine(p);
+    return engine == NLS_ENGINE_FUSED || engine == NLS_ENGINE_STAGE;
+}
+
+static int block_graph(const nls_problem *p, const void *P, void *psi, void *nres, void *ws, hipGraphExec_t *out) {
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "get device");
+    if (rc) return rc;
+    if (!g_graphs) g_graphs = new GraphCache;
+    GraphCache &gc = *g_graphs;
+    for (GraphEntry &e : gc.entries)
+        if (e.device == dev && e.P == P && e.psi == psi && e.nres == nres && e.ws == ws &&
+            memcmp(&e.p, p, sizeof *p) == 0) {
+            e.used = ++gc.clock;
+            *out = e.exec;
+            return NLS_OK;
+        }
+    if ((int)gc.capture.size() <= dev) gc.capture.resize(dev + 1, nullptr);
+    if (!gc.capture[dev]) {
+        rc = hip_check(hipStreamCreateWithFlags(&gc.capture[dev], hipStreamNonBlocking), "capture stream");
+        if (rc) return rc;
+    }
+    hipStream_t cs = gc.capture[dev];
+    rc = hip_check(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    rc = advance_direct(p, GRAPH_BLOCK, P, psi, nres, ws, cs);
+    hipGraph_t g = nullptr;
+    const int rc_end = hip_check(hipStreamEndCapture(cs, &g), "end capture");
+    if (rc || rc_end) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : rc_end;
+    }
+    hipGraphExec_t exec = nullptr;
+    rc = hip_check(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0), "instantiate graph");
+    (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    if (gc.entries.size() >= GRAPH_CACHE) {
+        // evict the least recently used entry; an earlier launch of it may
+        // still be in flight on any stream
+        size_t v = 0;
+        for (size_t i = 1; i < gc.entries.size(); ++i)
+            if (gc.entries[i].used < gc.entries[v].used) v = i;
+        rc = hip_check(hipDeviceSynchronize(), "evict graph");
+        if (rc) { (void)hipGraphExecDestroy(exec); return rc; }
+        (void)hipGraphExecDestroy(gc.entries[v].exec);
+        gc.entries.erase(gc.entries.begin() + v);
+    }
+    GraphEntry e;
+    e.device = dev;
+    e.p = *p;
+    e.P = P; e.psi = psi; e.nres = nres; e.ws = ws;
+    e.exec = exec;
+    e.used = ++gc.clock;
+    gc.entries.push_back(e);
+    *out = exec;
+    return NLS_OK;
+}
+
+int nls_release(void) {
+    if (!g_graphs) return NLS_OK;
+    int rc = hip_check(hipDeviceSynchronize(), "release");
+    for (GraphEntry &e : g_graphs->entries) (void)hipGraphExecDestroy(e.exec);
+    for (hipStream_t cs : g_graphs->capture)
+        if (cs) (void)hipStreamDestroy(cs);
+    delete g_graphs;
+    g_graphs = nullptr;
+    return rc;
 }
 
 int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *workspace,
@@ -195,12 +319,16 @@ int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi,
         return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
     if (iters == 0) return NLS_OK;
     hipStream_t s = (hipStream_t)stream;
-    if (p->scheme == NLS_SCHEME_SSF) return ssf_advance(p, iters, pumping, psi, nres, workspace, s);
-    if (use_fused(p)) return fused_grid_advance(p, iters, pumping, psi, nres, workspace, s);
-    if (p->dtype == NLS_F64)
-        return rk4_advance<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres,
-                                   (char *)workspace, s);
-    return rk4_advance<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)workspace, s);
+    if (!graphs_apply(p, iters)) return advance_direct(p, iters, pumping, psi, nres, workspace, s);
+    hipGraphExec_t exec = nullptr;
+    rc = block_graph(p, pumping, psi, dyn ? nres : nullptr, workspace, &exec);
+    if (rc) return rc;
+    for (int b = 0; b < iters / GRAPH_BLOCK; ++b) {
+        rc = hip_check(hipGraphLaunch(exec, s), "graph launch");
+        if (rc) return rc;
+    }
+    const int rest = iters % GRAPH_BLOCK;
+    return rest ? advance_direct(p, rest, pumping, psi, nres, workspace, s) : NLS_OK;
 }
 
 int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const void *nres, void *dpsi, void *dn,

@@ -12,12 +12,17 @@ int hip_check(hipError_t e, const char *what);     // NLS_OK or NLS_EHIP/NLS_ENO
 int check_problem(const nls_problem *p);
 size_t points_of(const nls_problem *p);
 
-bool use_fused(const nls_problem *p);              // plain RK4 solve on the fused engine
+int plain_engine(const nls_problem *p);            // RK4 engine of a plain solve (NLS_ENGINE_*)
 
 // plain 2-D RK4 solve of one grid on the fused kernel (nls_batch.hip)
 size_t fused_grid_workspace_bytes(const nls_problem *p);
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
                        hipStream_t s);
+// plain RK4 solve of one line (n <= 4096) or small grid (n <= 64) on the resident kernel
+bool resident_fits(const nls_problem *p);
+size_t resident_workspace_bytes(const nls_problem *p);
+int resident_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
+                     hipStream_t s);
 
 // split-step Fourier scheme (nls_ssf.hip)
 size_t ssf_workspace_bytes(const nls_problem *p);

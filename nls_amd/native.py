@@ -33,8 +33,10 @@ EXPORTS = (
     "nls_chemical_potential", "nls_observables",
     "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
     "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance", "nls_tune_stage_tile",
+    "nls_release",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2}
+FLAG_EAGER = 1
 
 
 class NativeError(RuntimeError):
@@ -44,21 +46,24 @@ class NativeError(RuntimeError):
 class ProblemDesc(C.Structure):
     """Mirror of ``nls_problem`` (include/nls_amd.h)."""
     _fields_ = [("dim", C.c_int32), ("n", C.c_int32), ("order", C.c_int32), ("bc", C.c_int32),
-                ("dtype", C.c_int32), ("scheme", C.c_int32), ("engine", C.c_int32), ("reserved", C.c_int32),
+                ("dtype", C.c_int32), ("scheme", C.c_int32), ("engine", C.c_int32), ("flags", C.c_int32),
                 ("dt", C.c_double), ("dx", C.c_double), ("coeffs", C.c_double * 23)]
 
     @classmethod
-    def make(cls, dim, n, order, dt, dx, coeffs, bc="ref", dtype="f32", scheme="rk4", engine="auto"):
-        """engine: RK4 engine of a plain solve -- "auto" (fused for 2-D,
-        stage for 1-D), "stage" (one launch per RK4 stage) or "fused"
-        (one launch per step, 2-D only)."""
+    def make(cls, dim, n, order, dt, dx, coeffs, bc="ref", dtype="f32", scheme="rk4", engine="auto",
+             graphs=True):
+        """engine: RK4 engine of a plain solve -- "auto" (resident when the
+        instance fits on chip, else fused for 2-D and stage for 1-D),
+        "stage" (one launch per RK4 stage), "fused" (one launch per step,
+        2-D only) or "resident" (1-D n <= 4096, 2-D n <= 64).  graphs=False
+        sets NLS_FLAG_EAGER: no step-block graph replay."""
         p = cls()
         p.dim, p.n, p.order = int(dim), int(n), int(order)
         p.bc = BC[bc] if isinstance(bc, str) else int(bc)
         p.dtype = DTYPE[dtype] if isinstance(dtype, str) else int(dtype)
         p.scheme = SCHEME[scheme] if isinstance(scheme, str) else int(scheme)
         p.engine = ENGINE[engine] if isinstance(engine, str) else int(engine)
-        p.reserved = 0
+        p.flags = 0 if graphs else FLAG_EAGER
         p.dt, p.dx = float(dt), float(dx)
         c = np.zeros(23)
         cc = np.asarray(coeffs, np.float64).ravel()
@@ -118,6 +123,7 @@ def load():
         "nls_solve_host": ([pp, i, vp, vp, vp], i),
         "nls_workspace_bytes": ([pp], sz),
         "nls_advance": ([pp, i, vp, vp, vp, vp, sz, vp], i),
+        "nls_release": ([], i),
         "nls_rhs": ([pp, vp, vp, vp, vp, vp, vp], i),
         "nls_chemical_potential": ([pp, vp, vp, vp, vp, vp], i),
         "nls_observables": ([pp, vp, vp, vp, vp], i),
@@ -276,6 +282,11 @@ def advance(desc, iters, pumping, psi, nres, workspace, stream=None):
     _check(load().nls_advance(C.byref(desc), int(iters), _dp(pumping), _dp(psi), _dp(nres),
                               _dp(workspace), workspace.numel() * workspace.element_size() if workspace is not None else 0,
                               C.c_void_p(s.cuda_stream)))
+
+
+def release():
+    """Free the calling thread's cached step-block graphs (nls_release)."""
+    _check(load().nls_release())
 
 
 def rhs(desc, pumping, psi, nres, dpsi, dn, stream=None):

@@ -70,7 +70,7 @@ def test_workspace_bytes():
     c[10] = 1.0
     d = native.ProblemDesc.make(2, 256, 5, 1e-3, 0.1, c, bc="periodic", dtype="f64", engine="stage")
     assert native.workspace_bytes(d) >= w_adi + 3 * 256 * 256 * 8
-    # fused engine (the 2-D default): the other half of the state ping-pong
+    # fused engine (the 2-D default above 64 x 64): the other half of the state ping-pong
     for engine in ("auto", "fused"):
         d = native.ProblemDesc.make(2, 256, 5, 1e-3, 0.1, c, bc="periodic", dtype="f64", engine=engine)
         assert 256 * 256 * 24 <= native.workspace_bytes(d) < 256 * 256 * 24 + 1024
@@ -81,7 +81,10 @@ def test_engine_field():
     with pytest.raises(ValueError):       # the fused engine is 2-D only
         native.solve_host(native.ProblemDesc.make(1, 64, 5, 1e-3, 0.1, c, engine="fused"), 1,
                           np.ones(64), np.ones(64))
-    with pytest.raises(ValueError):       # batch-only engine
-        native.solve_host(native.ProblemDesc.make(2, 16, 5, 1e-3, 0.1, c, engine="resident"), 1,
+    with pytest.raises(ValueError):       # the resident engine holds 2-D n <= 64 only
+        native.solve_host(native.ProblemDesc.make(2, 128, 5, 1e-3, 0.1, c, engine="resident"), 1,
+                          np.ones((128, 128)), np.ones((128, 128)))
+    with pytest.raises(ValueError):
+        native.solve_host(native.ProblemDesc.make(2, 16, 5, 1e-3, 0.1, c, engine=7), 1,
                           np.ones((16, 16)), np.ones((16, 16)))
     assert C.sizeof(native.ProblemDesc) == 8 * 4 + 2 * 8 + 23 * 8

@@ -1,0 +1,103 @@
+"""Step-block graphs of nls_advance.
+
+A solve of >= 128 steps on the fused or per-stage engine replays a captured
+block of 64 steps as one graph launch.  The replayed result must equal the
+result of issuing every launch directly (bitwise), on every replay -- the
+test replays the cached graph several times with direct work and stream
+synchronisations in between -- and match the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from nls_amd import configs, native
+
+pytestmark = pytest.mark.gpu
+
+
+def setup(dim, n, dtype, engine, dyn=True, seed=5):
+    import torch
+    rng = np.random.default_rng(seed)
+    if dim == 1:
+        x, h = configs.grid_1d(n, 0.1 * n)
+        r2 = x ** 2
+        shape = (n,)
+    else:
+        X, Y, h = configs.grid_2d(n, 10.0)
+        r2 = X ** 2 + Y ** 2
+        shape = (n, n)
+    psi0 = 0.3 * np.exp(-r2 / 4.0) + configs.noise(rng, shape, 1e-2)
+    P = 2.0 * np.exp(-r2 / 6.0)
+    n0 = 0.2 * np.exp(-r2 / 3.0)
+    c = configs.model_coeffs(dynamic=dyn)
+    dt = 1e-3 if dim == 2 else 5e-4
+    kw = dict(bc="periodic", dtype=dtype, engine=engine)
+    d_graph = native.ProblemDesc.make(dim, n, 5, dt, h, c, **kw)
+    d_eager = native.ProblemDesc.make(dim, n, 5, dt, h, c, graphs=False, **kw)
+    rt = torch.float64 if dtype == "f64" else torch.float32
+    ct = torch.complex128 if dtype == "f64" else torch.complex64
+    dev = torch.device("cuda:0")
+    Pt = torch.as_tensor(P, dtype=rt, device=dev)
+    state = lambda: (torch.as_tensor(psi0, dtype=ct, device=dev).clone(),
+                     torch.as_tensor(n0, dtype=rt, device=dev).clone())
+    ws = torch.empty(max(1, native.workspace_bytes(d_graph)), dtype=torch.uint8, device=dev)
+    ws2 = torch.empty(max(1, native.workspace_bytes(d_eager)), dtype=torch.uint8, device=dev)
+    return d_graph, d_eager, Pt, state, ws, ws2, (P, psi0, n0, c, dt, h)
+
+
+@pytest.mark.parametrize("dim,n,engine", [(2, 48, "fused"), (2, 40, "stage"), (1, 300, "stage"), (2, 130, "fused")])
+@pytest.mark.parametrize("dtype", ("f64", "f32"))
+def test_graph_replays_equal_direct_launches(gpu, dim, n, engine, dtype):
+    import torch
+    d_graph, d_eager, Pt, state, ws, ws2, _ = setup(dim, n, dtype, engine)
+    steps = 200                                   # 3 graph blocks + 8 direct steps
+    psi_e, n_e = state()
+    native.advance(d_eager, steps, Pt, psi_e, n_e, ws2)
+    torch.cuda.synchronize()
+    ref_psi, ref_n = psi_e.cpu().numpy(), n_e.cpu().numpy()
+    for rep in range(3):                          # first call captures, later ones hit the cache
+        psi_g, n_g = state()
+        native.advance(d_graph, steps, Pt, psi_g, n_g, ws)
+        # direct work on the same stream between the replays
+        psi_x, n_x = state()
+        native.advance(d_eager, 5, Pt, psi_x, n_x, ws2)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(psi_g.cpu().numpy(), ref_psi)
+        np.testing.assert_array_equal(n_g.cpu().numpy(), ref_n)
+        # the same buffers again: the state continues from where it stopped
+        native.advance(d_graph, steps, Pt, psi_g, n_g, ws)
+        native.advance(d_eager, steps, Pt, psi_e, n_e, ws2)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(psi_g.cpu().numpy(), psi_e.cpu().numpy())
+        psi_e, n_e = state()
+        native.advance(d_eager, steps, Pt, psi_e, n_e, ws2)
+        torch.cuda.synchronize()
+    native.release()
+
+
+def test_graph_solve_matches_oracle(gpu, oracle):
+    import torch
+    d_graph, _, Pt, state, ws, _, (P, psi0, n0, c, dt, h) = setup(2, 48, "f64", "fused")
+    psi, nres = state()
+    native.advance(d_graph, 300, Pt, psi, nres, ws)
+    torch.cuda.synchronize()
+    opsi, onres = oracle.rk4(2, 48, 5, 1, dt, h, 300, P, c, psi0, n0)
+    rel = np.linalg.norm(psi.cpu().numpy() - opsi) / np.linalg.norm(opsi)
+    assert rel <= 1e-10, rel
+    reln = np.linalg.norm(nres.cpu().numpy() - onres) / np.linalg.norm(onres)
+    assert reln <= 1e-10, reln
+
+
+def test_graph_cache_eviction(gpu):
+    """More distinct buffer sets than cache entries: every solve stays right."""
+    import torch
+    d_graph, d_eager, Pt, state, ws, ws2, _ = setup(2, 40, "f32", "fused")
+    psi_e, n_e = state()
+    native.advance(d_eager, 130, Pt, psi_e, n_e, ws2)
+    torch.cuda.synchronize()
+    ref = psi_e.cpu().numpy()
+    for k in range(10):
+        psi, nres = state()
+        native.advance(d_graph, 130, Pt, psi, nres, ws)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(psi.cpu().numpy(), ref)
+    native.release()
