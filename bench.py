@@ -10,7 +10,10 @@ Reported value: grid-point-steps/s of the batched complex64 solve (FP32
 arithmetic) with all inputs resident in HBM.  Beside it: the plain FP64 path
 (one instance at a time) on a sample of the same instances, the relative L2
 error of the batched FP32 result against that FP64 result after the full step
-count, the roofline of the dominant kernel, and the CPU baseline.
+count, the roofline of the dominant kernel, the CPU baseline, one 2048 x 2048
+grid on the plain path (FP64 and FP32, both RK4 engines, split-step Fourier)
+and the continuous-batching request loop (p50/p99 latency under Poisson
+arrivals).
 
   python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -81,13 +84,79 @@ def cpu_baseline(members, steps):
                 "sample": sample + "; C restatement, FP32, same boundaries and reservoir"}
 
 
+def single_grid_2048(dev, steps=100):
+    """One 2048 x 2048 periodic grid (h = 0.0625, order 5) on the plain path:
+    FP64 and FP32, dynamic reservoir, each RK4 engine and the split-step
+    Fourier scheme.  Rates in grid-point-steps/s; the HBM fraction of each is
+    given on two bases: the per-stage (k-storage) scheme's 15 S + 4 r bytes
+    per point-step, whose 8 TB/s ceiling is the reference roofline of this
+    path, and the compulsory 2 S + r (SSF: 4 c + 3 r)."""
+    import torch
+    from nls_amd import configs, native
+    n = 2048
+    out = {}
+    for dtype in ("f64", "f32"):
+        cs, rs = (16, 8) if dtype == "f64" else (8, 4)
+        S = cs + rs
+        rt = torch.float64 if dtype == "f64" else torch.float32
+        ct = torch.complex128 if dtype == "f64" else torch.complex64
+        P = torch.ones((n, n), dtype=rt, device=dev)
+        for name, kw, kbytes, mbytes in (("rk4_fused", dict(engine="fused"), 15 * S + 4 * rs, 2 * S + rs),
+                                          ("rk4_stage", dict(engine="stage"), 15 * S + 4 * rs, 2 * S + rs),
+                                          ("ssf", dict(scheme="ssf"), None, 4 * cs + 3 * rs)):
+            d = native.ProblemDesc.make(2, n, 5, 1e-4, 0.0625, configs.model_coeffs(dynamic=True), bc="periodic",
+                                        dtype=dtype, **kw)
+            psi = torch.full((n, n), 0.1, dtype=ct, device=dev)
+            nres = torch.zeros((n, n), dtype=rt, device=dev)
+            ws = torch.empty(max(1, native.workspace_bytes(d)), dtype=torch.uint8, device=dev)
+            native.advance(d, 4, P, psi, nres, ws)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            native.advance(d, steps, P, psi, nres, ws)
+            torch.cuda.synchronize()
+            t = time.perf_counter() - t0
+            rate = n * n * steps / t
+            r = {"grid_point_steps_per_s": rate, "us_per_step": t / steps * 1e6,
+                 "hbm_frac_compulsory_bytes": rate * mbytes / HBM_PEAK}
+            if kbytes:
+                r["hbm_frac_kstorage_bytes"] = rate * kbytes / HBM_PEAK
+                r["vs_kstorage_ceiling"] = rate / (HBM_PEAK / kbytes)
+            out["%s_%s" % (name, dtype)] = r
+            log("2048^2 %s %s: %.1f us/step, %.2f G grid-point-steps/s" % (name, dtype, t / steps * 1e6, rate / 1e9))
+    return out
+
+
+def serving(rate, window, seed=0):
+    """Continuous batching: Poisson arrivals at `rate` per second over `window`
+    seconds, the request mix of nls_amd.serving.make_request; latency is
+    arrival to result on the host."""
+    from nls_amd import serving as S
+    import torch
+    eng = S.Engine(budget_bytes=16 << 30)
+    reqs = S.poisson_requests(rate, window, seed=seed)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = eng.run(reqs)
+    wall = time.perf_counter() - t0
+    eng.close()
+    s = S.summarize(done, wall)
+    s.update({"rate_per_s": rate, "arrival_window_s": window, "wall_s": wall})
+    log("serving at %g req/s: %d requests, p50 %.3fs, p99 %.3fs" % (rate, s["requests"], s["p50_s"], s["p99_s"]))
+    return s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison")
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison and the 2048^2 grid")
+    ap.add_argument("--no-serve", action="store_true", help="skip the request-loop latency measurement")
+    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-window", type=float, default=2.0, help="arrival window (s)")
+    ap.add_argument("--grid2048-only", action="store_true",
+                    help="only measure the 2048 x 2048 single grid (plain path) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -111,6 +180,10 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copied %d bytes 5 times" % (src.numel() * 4))
+        return
+
+    if args.grid2048_only:
+        print(json.dumps({"single_grid_2048": single_grid_2048(dev)}))
         return
 
     t0 = time.perf_counter()
@@ -190,11 +263,14 @@ def main():
         out["speedup_vs_plain_f64"] = value / plain
         out["accuracy"] = {"max_relL2_vs_f64": max(errs.values()), "per_size": {str(k): v for k, v in errs.items()},
                            "tolerance": 1e-4}
+        out["single_grid_2048"] = single_grid_2048(dev)
 
+    batch.close()
+    if not args.no_serve:
+        out["serving"] = serving(args.serve_rate, args.serve_window)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(members[:16], 200)
         log("cpu baseline: %.3g grid-point-steps/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["kind"]))
-    batch.close()
     print(json.dumps(out))
 
 
