@@ -155,6 +155,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop latency measurement")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-window", type=float, default=2.0, help="arrival window (s)")
+    ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "stage", "resident", "auto"),
+                    help="batch engine of the headline workload (tuned: measured on this device)")
     ap.add_argument("--grid2048-only", action="store_true",
                     help="only measure the 2048 x 2048 single grid (plain path) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -191,7 +193,17 @@ def main():
     points = sum(m.points for m in members)
     log("sweep512: %d instances, %d points, generated in %.1fs" % (len(members), points, time.perf_counter() - t0))
 
-    batch = Batch(members, dim=2, order=5, bc="periodic", dtype="f32", engine="fused", device=dev)
+    # engine of the batch: measured on this device among those that can hold
+    # the member mix (nls_amd.autotune), unless given
+    tuning = None
+    engine = args.engine
+    if engine == "tuned":
+        from nls_amd.autotune import Tuner
+        tuner = Tuner(path=None, steps=32, reps=2)
+        engine = tuner.batch_engine(members, dim=2, order=5, bc="periodic", dtype="f32")
+        tuning = {k: v * 1e6 for k, v in tuner.last.items()}
+        log("tuned batch engine: %s (%s us/step)" % (engine, ", ".join("%s %.1f" % kv for kv in tuning.items())))
+    batch = Batch(members, dim=2, order=5, bc="periodic", dtype="f32", engine=engine, device=dev)
     stream = torch.cuda.current_stream()
     if W > 0:
         batch.advance(W)
@@ -214,21 +226,31 @@ def main():
     launch_s = gpu_s / K             # one fused-kernel launch per step; events on its stream
     log("batched FP32: %.3fs wall, %.1f us/step, %.2f G grid-point-steps/s" % (wall, wall / K * 1e6, value / 1e9))
 
-    bytes_per_launch = points * FUSED_BYTES_F32_DYN
-    achieved = bytes_per_launch / launch_s
-    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK, "traffic": read_traffic("fused_step_f32_sweep512_traffic.txt"),
-                "kernel": "k_fused_step_2d<float,5,periodic,dynamic>",
-                "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6}
+    if engine == "fused":
+        bytes_per_launch = points * FUSED_BYTES_F32_DYN
+        achieved = bytes_per_launch / launch_s
+        roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK, "traffic": read_traffic("fused_step_f32_sweep512_traffic.txt"),
+                    "kernel": "k_fused_step_2d<float,5,periodic,dynamic>",
+                    "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6}
+    else:
+        # per-stage engine: four launches per step, 15 S + 4 r bytes per point-step
+        bytes_per_step = points * (15 * 12 + 4 * 4)
+        achieved = bytes_per_step / launch_s
+        roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                    "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_batch_stage (4 per step)",
+                    "algorithmic_bytes_per_step": bytes_per_step, "avg_step_us": launch_s * 1e6}
 
     out = {"metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
            "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": "sweep512: 512 2-D instances, sides 64/128/256 (p=0.5/0.3/0.2), periodic, "
                                   "order 5, dynamic reservoir, dt=5e-4, %d RK4 steps, complex64 state" % K,
-                      "instances": len(members), "grid_points": points, "engine": "fused",
+                      "instances": len(members), "grid_points": points, "engine": engine,
                       "parallelism": "single GPU"},
            "roofline": roofline}
+    if tuning is not None:
+        out["engine_tuning_us_per_step"] = tuning
 
     if not args.no_plain:
         # plain FP64 path, one instance at a time, on one instance of each size

@@ -15,6 +15,9 @@ Added keyword arguments of ``Problem().model``:
          operators) or 'periodic' (Cartesian stencils with wrap-around; the
          pumping and u0 grids are then x_k = -n dx/2 + k dx)
   scheme 'rk4' (default) or 'ssf' (split-step Fourier, periodic only)
+  engine RK4 engine: 'auto' (default), 'stage', 'fused', 'resident' or
+         'tuned' (the fastest on this device, measured once and cached by
+         nls_amd.autotune)
   coeffs an explicit 23-entry coefficient vector replacing the derived one
   n0     initial reservoir density when coeffs[10] != 0 (default zero)
 Plotting (model.py:391-498) is not part of this package.
@@ -125,6 +128,7 @@ class AbstractModel(object):
         self.dtype = kwargs.get("dtype", "f32")
         self.bc = kwargs.get("bc", "ref")
         self.scheme = kwargs.get("scheme", "rk4")
+        self.engine = kwargs.get("engine", "auto")
         self.init_res = kwargs.get("n0")
         self.solver = None
         self.coeffs = self.deriveCoefficients(self.originals)
@@ -158,7 +162,8 @@ class AbstractModel(object):
         from pprint import pformat
         return pformat({"dt": self.dt, "dx": self.dx, "order": self.order, "num_nodes": self.num_nodes,
                         "num_iters": self.num_iters, "pumping": self.pumping, "originals": self.originals,
-                        "dtype": self.dtype, "bc": self.bc, "scheme": self.scheme}) + "\n" + str(self.coeffs)
+                        "dtype": self.dtype, "bc": self.bc, "scheme": self.scheme,
+                        "engine": self.engine}) + "\n" + str(self.coeffs)
 
     def getApproximationOrder(self):
         return self.order
@@ -239,7 +244,7 @@ class AbstractModel(object):
 
     def store(self, filename=None, label=None, desc=None, date=None):
         """Write the model to a .mat file (model.py:257-285) plus the fields
-        of this package (dtype, bc, scheme, initial reservoir)."""
+        of this package (dtype, bc, scheme, engine, initial reservoir)."""
         from scipy.io import savemat
         date = date if date else datetime.now()
         date = date.replace(microsecond=0).isoformat()
@@ -248,7 +253,7 @@ class AbstractModel(object):
                "dimlesses": self.coeffs, "init_solution": self.init_sol, "num_iters": self.num_iters,
                "num_nodes": self.num_nodes, "order": self.order, "originals": self.originals,
                "pumping": self.getPumping(), "spatial_step": self.dx, "time_step": self.dt,
-               "dtype": self.dtype, "bc": self.bc, "scheme": self.scheme}
+               "dtype": self.dtype, "bc": self.bc, "scheme": self.scheme, "engine": self.engine}
         if self.init_res is not None:
             mat["init_reservoir"] = self.init_res
         if desc:
@@ -276,7 +281,7 @@ class AbstractModel(object):
         self.dt = float(np.asarray(mat["time_step"]).ravel()[0])
         o = mat["originals"][0, 0]
         self.originals = {name: float(np.asarray(o[name]).ravel()[0]) for name in o.dtype.names}
-        for key in ("dtype", "bc", "scheme"):
+        for key in ("dtype", "bc", "scheme", "engine"):
             if key in mat:
                 setattr(self, key, str(np.asarray(mat[key]).ravel()[0]))
         if "init_reservoir" in mat:

@@ -66,12 +66,18 @@ class AbstractSolver(object):
     def _reference_mode(self, coeffs):
         m = self.model
         return (getattr(m, "dtype", "f32") == "f32" and getattr(m, "bc", "ref") == "ref"
-                and getattr(m, "scheme", "rk4") == "rk4" and np.asarray(coeffs)[10] == 0.0)
+                and getattr(m, "scheme", "rk4") == "rk4" and getattr(m, "engine", "auto") == "auto"
+                and np.asarray(coeffs)[10] == 0.0)
 
     def _solve_extended(self, dim, dt, dx, order, iters, pumping, coeffs, u0):
         m = self.model
+        engine = getattr(m, "engine", "auto")
         desc = native.ProblemDesc.make(dim, m.getNumberOfNodes(), order, dt, dx, coeffs, bc=m.bc,
-                                       dtype=m.dtype, scheme=m.scheme)
+                                       dtype=m.dtype, scheme=m.scheme,
+                                       engine="auto" if engine == "tuned" else engine)
+        if engine == "tuned":
+            from .autotune import Tuner
+            desc = Tuner().tuned(desc)
         psi, nres = native.solve_host(desc, iters, pumping, u0, m.getInitialReservoir())
         return (psi, nres) if nres is not None else psi
 

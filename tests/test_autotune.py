@@ -1,0 +1,78 @@
+"""The autotuner (nls_amd/autotune.py): candidate sets, choice of the fastest,
+the on-disk cache.  CPU tests inject the timer; the GPU test times the real
+engines and checks that the tuned solve still matches the oracle."""
+import json
+
+import numpy as np
+import pytest
+
+from nls_amd import autotune, configs, native
+
+
+def fake_timer(table, calls):
+    def t(cand, _setup):
+        calls.append(cand)
+        return table[cand[1]]
+    return t
+
+
+def test_plain_candidates_follow_what_fits():
+    T = autotune.Tuner(path=None, device_name="dev")
+    c = configs.model_coeffs(dynamic=True)
+    assert T.plain_candidates(native.ProblemDesc.make(2, 64, 5, 1e-3, 0.1, c)) == ["resident", "fused", "stage"]
+    assert T.plain_candidates(native.ProblemDesc.make(2, 128, 5, 1e-3, 0.1, c)) == ["fused", "stage"]
+    assert T.plain_candidates(native.ProblemDesc.make(2, 64, 7, 1e-3, 0.1, c, dtype="f64")) == ["fused", "stage"]
+    assert T.plain_candidates(native.ProblemDesc.make(1, 4096, 5, 1e-3, 0.1, c)) == ["resident", "stage"]
+    assert T.plain_candidates(native.ProblemDesc.make(1, 5000, 5, 1e-3, 0.1, c)) == ["stage"]
+
+
+def test_fastest_candidate_is_chosen_and_cached(tmp_path):
+    path = str(tmp_path / "tune.json")
+    calls = []
+    T = autotune.Tuner(path=path, reps=2, timer=fake_timer({"resident": 3e-6, "fused": 1e-6, "stage": 2e-6}, calls),
+                       device_name="dev")
+    d = native.ProblemDesc.make(2, 64, 5, 1e-3, 0.1, configs.model_coeffs(dynamic=True))
+    assert T.plain_engine(d) == "fused"
+    assert len(calls) == 6                       # 3 candidates x 2 repetitions
+    assert T.tuned(d).engine == native.ENGINE["fused"]
+    # a new tuner reads the choice from disk and measures nothing
+    calls2 = []
+    T2 = autotune.Tuner(path=path, timer=fake_timer({"resident": 0, "fused": 1, "stage": 1}, calls2), device_name="dev")
+    assert T2.plain_engine(d) == "fused" and calls2 == []
+    # another device or problem class is measured anew
+    T3 = autotune.Tuner(path=path, timer=fake_timer({"resident": 0, "fused": 1, "stage": 1}, calls2), device_name="other")
+    assert T3.plain_engine(d) == "resident" and len(calls2) == 9
+    table = json.load(open(path))
+    assert len(table) == 2 and all("seconds_per_step" in v for v in table.values())
+
+
+def test_stage_tile_and_batch_choices():
+    calls = []
+    T = autotune.Tuner(path=None, timer=lambda cand, _s: {8: 3.0, 16: 1.0, 32: 2.0}[cand[1]], device_name="dev")
+    assert T.stage_tile("f32") == 16
+    ms = configs.sweep512(count=4, steps=10)
+    T = autotune.Tuner(path=None, timer=fake_timer({"resident": 9, "fused": 1, "stage": 2}, calls), device_name="dev")
+    assert T.batch_engine(ms) == "fused"
+    assert all(c[1] != "resident" for c in calls) or all(m.n == 64 for m in ms)
+
+
+@pytest.mark.gpu
+def test_tuned_plain_solve_matches_oracle(gpu, oracle, tmp_path):
+    T = autotune.Tuner(path=str(tmp_path / "t.json"), steps=64, reps=1)
+    n = 48
+    X, Y, h = configs.grid_2d(n, 10.0)
+    r2 = X ** 2 + Y ** 2
+    rng = np.random.default_rng(3)
+    psi0 = 0.3 * np.exp(-r2 / 4.0) + configs.noise(rng, (n, n), 1e-2)
+    P = 2.0 * np.exp(-r2 / 6.0)
+    n0 = 0.2 * np.exp(-r2 / 3.0)
+    c = configs.model_coeffs(dynamic=True)
+    d = native.ProblemDesc.make(2, n, 5, 1e-3, h, c, bc="periodic", dtype="f64")
+    engine = T.plain_engine(d)
+    assert engine in ("resident", "fused", "stage") and set(T.last) == {"resident", "fused", "stage"}
+    psi, nres = native.solve_host(T.tuned(d), 200, P, psi0, n0)
+    opsi, _ = oracle.rk4(2, n, 5, 1, 1e-3, h, 200, P, c, psi0, n0)
+    assert np.linalg.norm(psi - opsi) / np.linalg.norm(opsi) <= 1e-10
+    rows = T.stage_tile("f64", n=256)
+    assert rows in (8, 16, 32) and native.stage_tile("f64") == rows
+    native.stage_tile("f64", 16)
