@@ -9,11 +9,15 @@ schedulr/workr.py:39-107).  Here requests share the GPU:
   that do not fit wait in arrival order, later ones that fit go ahead;
 * the state stays on the device for the whole solve; inputs are uploaded
   once at admission and the result downloaded once at completion;
-* every block advances all active requests together -- 2-D requests in one
-  batch on the fused RK4 engine, 1-D requests in one batch on the stage
-  engine -- each by min(block, remaining) steps (nls_batch_set_steps), so a
-  request that finishes mid-block holds its state and its slot is freed at
-  the end of the block, when waiting requests are admitted;
+* three pools, each one batch: 1-D requests (n <= 4096) and 2-D requests up
+  to 64 x 64 on the resident engine (a workgroup per request runs every step
+  of a block in one launch), larger 2-D requests on the fused RK4 engine (one
+  launch per step); the HBM budget is split 15% / 15% / 70%;
+* every block advances all active requests together, each by
+  min(block, remaining) steps (nls_batch_set_steps), so a request that
+  finishes mid-block holds its state and its slot is freed at the end of the
+  block, when waiting requests are admitted; the block length adapts so one
+  block takes about target_block_ms;
 * a member's arithmetic never depends on the other members, so a result is
   bit-identical alone and in any mix.
 
