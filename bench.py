@@ -128,12 +128,13 @@ def single_grid_2048(dev, steps=100):
 
 def serving(rate, window, seed=0):
     """Continuous batching: Poisson arrivals at `rate` per second over `window`
-    seconds, the request mix of nls_amd.serving.make_request; latency is
+    seconds, the request mix of nls_amd.serving.make_request with inputs given
+    by parameters and generated on the device at admission; latency is
     arrival to result on the host."""
     from nls_amd import serving as S
     import torch
     eng = S.Engine(budget_bytes=16 << 30)
-    reqs = S.poisson_requests(rate, window, seed=seed)
+    reqs = S.poisson_requests(rate, window, seed=seed, device_inputs=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     done = eng.run(reqs)

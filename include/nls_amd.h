@@ -168,6 +168,51 @@ int nls_chemical_potential(const nls_problem *p, const void *pumping, const void
 int nls_observables(const nls_problem *p, const void *psi, const void *nres, double *out, void *stream);
 
 /* ======================================================================
+ * Input fields generated on the device: pumping profiles and initial
+ * states sampled on a grid, so inputs given by parameters need no
+ * host-to-device upload.  Profiles follow the reference's functors
+ * (nls/pumping.py:113-178) sampled like its models (nls/model.py:220-232);
+ * values are formed in double precision in numpy's order (no fused
+ * multiply-add) and rounded once to dtype.
+ * ====================================================================== */
+
+/* Grid along each axis (2-D: the same grid for x and y; element [row, col]
+ * sits at (x[col], x[row]) like numpy.meshgrid(x, x)). */
+#define NLS_GRID_LINSPACE  0  /* numpy.linspace(a, b, n): the reference's grids */
+#define NLS_GRID_STEP      1  /* a + b*k: the periodic model grid           */
+#define NLS_GRID_FRACTION  2  /* a + (b*k)/n: a period b split in n cells    */
+typedef struct nls_grid {
+    int32_t kind;
+    int32_t reserved;
+    double a, b;
+} nls_grid;
+
+#define NLS_PROFILE_CONSTANT      0  /* amplitude                                   */
+#define NLS_PROFILE_GAUSSIAN      1  /* GaussianPumping1D/2D (pumping.py:113-149)    */
+#define NLS_PROFILE_GAUSSIAN_RING 2  /* GaussianRingPumping1D/2D (pumping.py:152-178):
+                                        Gaussians at +radius and -radius of x (1-D)
+                                        or of r = |(x,y) - (x0,y0)| (2-D)          */
+#define NLS_PROFILE_RING          3  /* one Gaussian at r = radius (no mirror term)  */
+typedef struct nls_profile {
+    int32_t kind;
+    int32_t reserved;
+    double amplitude;   /* power                                           */
+    double x0, y0;      /* centre                                          */
+    double width;       /* variation (standard deviation)                  */
+    double radius;
+} nls_profile;
+
+/* out[n] (1-D) or out[n*n] (2-D), real dtype, on the device. */
+int nls_fill_profile(const nls_profile *profile, int dim, int n, const nls_grid *grid, int dtype, void *out,
+                     void *stream);
+
+/* psi (complex dtype) = profile + noise*(z_re + i z_im)/sqrt(2) with
+ * standard normal z from a counter-based generator keyed by seed and the
+ * element index (reproducible, independent of launch shape). */
+int nls_fill_state(const nls_profile *profile, double noise, uint64_t seed, int dim, int n, const nls_grid *grid,
+                   int dtype, void *psi, void *stream);
+
+/* ======================================================================
  * Batched solves: many independent instances ("members") of mixed sizes in
  * shared packed device arrays, advanced together.  Members share dim,
  * order, boundary family, reservoir mode and dtype; each has its own n, dt,

@@ -33,7 +33,7 @@ EXPORTS = (
     "nls_chemical_potential", "nls_observables",
     "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
     "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance", "nls_tune_stage_tile",
-    "nls_release",
+    "nls_release", "nls_fill_profile", "nls_fill_state",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2}
 FLAG_EAGER = 1
@@ -41,6 +41,17 @@ FLAG_EAGER = 1
 
 class NativeError(RuntimeError):
     """A HIP/runtime failure reported by libnls_amd."""
+
+
+class GridDesc(C.Structure):
+    """Mirror of ``nls_grid`` (include/nls_amd.h)."""
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("a", C.c_double), ("b", C.c_double)]
+
+
+class ProfileDesc(C.Structure):
+    """Mirror of ``nls_profile`` (include/nls_amd.h)."""
+    _fields_ = [("kind", C.c_int32), ("reserved", C.c_int32), ("amplitude", C.c_double), ("x0", C.c_double),
+                ("y0", C.c_double), ("width", C.c_double), ("radius", C.c_double)]
 
 
 class ProblemDesc(C.Structure):
@@ -124,6 +135,8 @@ def load():
         "nls_workspace_bytes": ([pp], sz),
         "nls_advance": ([pp, i, vp, vp, vp, vp, sz, vp], i),
         "nls_release": ([], i),
+        "nls_fill_profile": ([C.POINTER(ProfileDesc), i, i, C.POINTER(GridDesc), i, vp, vp], i),
+        "nls_fill_state": ([C.POINTER(ProfileDesc), d, C.c_uint64, i, i, C.POINTER(GridDesc), i, vp, vp], i),
         "nls_rhs": ([pp, vp, vp, vp, vp, vp, vp], i),
         "nls_chemical_potential": ([pp, vp, vp, vp, vp, vp], i),
         "nls_observables": ([pp, vp, vp, vp, vp], i),

@@ -8,7 +8,9 @@ schedulr/workr.py:39-107).  Here requests share the GPU:
   (state psi, reservoir n, pumping P) from a first-fit allocator; requests
   that do not fit wait in arrival order, later ones that fit go ahead;
 * the state stays on the device for the whole solve; inputs are uploaded
-  once at admission and the result downloaded once at completion;
+  once at admission -- or, for requests given by parameters, generated in
+  place on the device (nls_amd.fields) -- and the result downloaded once at
+  completion;
 * three pools, each one batch: 1-D requests (n <= 4096) and 2-D requests up
   to 64 x 64 on the resident engine (a workgroup per request runs every step
   of a block in one launch), larger 2-D requests on the fused RK4 engine (one
@@ -29,15 +31,21 @@ import time
 
 import numpy as np
 
-from . import configs, native
+from . import configs, fields, native
 
 ALIGN = 64          # points; members start on 64-point boundaries
 
 
 class Request(object):
-    def __init__(self, rid, arrival, dim, n, steps, dt, dx, coeffs, P, psi0, n0):
+    """One solve.  Inputs either as host arrays (P, psi0, n0) or as a `spec`
+    of parameters generated on the device at admission (nls_amd.fields):
+    dict(grid=Grid, pump=Profile, state=Profile, noise=std, seed=int); the
+    reservoir then starts at zero."""
+
+    def __init__(self, rid, arrival, dim, n, steps, dt, dx, coeffs, P=None, psi0=None, n0=None, spec=None):
         self.rid, self.arrival, self.dim, self.n, self.steps = rid, arrival, dim, n, steps
         self.dt, self.dx, self.coeffs, self.P, self.psi0, self.n0 = dt, dx, coeffs, P, psi0, n0
+        self.spec = spec
         self.remaining = steps
         self.offset = None
         self.admit_time = self.finish_time = None
@@ -119,10 +127,20 @@ class Group(object):
         a, b = off, off + req.points
         t = self.torch
         ct = self.psi.dtype
-        self.psi[a:b] = t.from_numpy(np.ascontiguousarray(req.psi0.ravel())).to(self.device, ct, non_blocking=True)
-        self.P[a:b] = t.from_numpy(np.ascontiguousarray(req.P.ravel())).to(self.device, self.P.dtype, non_blocking=True)
-        n0 = np.zeros(req.points) if req.n0 is None else req.n0.ravel()
-        self.nres[a:b] = t.from_numpy(np.ascontiguousarray(n0)).to(self.device, self.nres.dtype, non_blocking=True)
+        if req.spec is not None:
+            # inputs generated in place on the device: no upload
+            sp = req.spec
+            fields.fill_profile(self.P[a:b], sp["pump"], req.dim, req.n, sp["grid"])
+            fields.fill_state(self.psi[a:b], sp["state"], sp["noise"], sp["seed"], req.dim, req.n, sp["grid"])
+            self.nres[a:b].zero_()
+        else:
+            self.psi[a:b] = t.from_numpy(np.ascontiguousarray(req.psi0.ravel())).to(self.device, ct,
+                                                                                    non_blocking=True)
+            self.P[a:b] = t.from_numpy(np.ascontiguousarray(req.P.ravel())).to(self.device, self.P.dtype,
+                                                                              non_blocking=True)
+            n0 = np.zeros(req.points) if req.n0 is None else req.n0.ravel()
+            self.nres[a:b] = t.from_numpy(np.ascontiguousarray(n0)).to(self.device, self.nres.dtype,
+                                                                        non_blocking=True)
         self.members.append(req)
         self.dirty = True
         return True
@@ -276,7 +294,13 @@ def stable_dt(dim, h, order=5, safety=0.5):
     return safety * 2.0 * math.sqrt(2.0) / (dim * sym / (h * h))
 
 
-def make_request(rid, arrival, rng, step_range=(1e3, 1e5)):
+def make_request(rid, arrival, rng, step_range=(1e3, 1e5), device_inputs=False):
+    """A request of the serving mix.  device_inputs: give the pumping and the
+    initial state as parameters (Gaussian or single-Gaussian ring pumping,
+    0.1 exp(-r^2/4) plus seeded noise of std 1e-3), generated on the device
+    at admission, instead of host arrays."""
+    if device_inputs:
+        return _make_spec_request(rid, arrival, rng, step_range)
     dim = 1 if rng.random() < 0.5 else 2
     if dim == 1:
         n = int(round(math.exp(rng.uniform(math.log(256), math.log(4096))) / 64.0)) * 64
@@ -297,14 +321,33 @@ def make_request(rid, arrival, rng, step_range=(1e3, 1e5)):
                    np.zeros(r.shape))
 
 
-def poisson_requests(rate, duration, seed=0, step_range=(1e3, 1e5)):
+def _make_spec_request(rid, arrival, rng, step_range):
+    dim = 1 if rng.random() < 0.5 else 2
+    if dim == 1:
+        n = int(round(math.exp(rng.uniform(math.log(256), math.log(4096))) / 64.0)) * 64
+        L = 64.0
+    else:
+        n = int(round(math.exp(rng.uniform(math.log(64), math.log(512))) / 64.0)) * 64
+        L = 32.0
+    h = L / n
+    steps = int(round(math.exp(rng.uniform(math.log(step_range[0]), math.log(step_range[1])))))
+    P0, r0, w = rng.uniform(0.5, 5.0), rng.uniform(3.0, 8.0), rng.uniform(0.5, 2.0)
+    pump = (fields.Profile("gaussian", P0, width=w) if rng.random() < 0.5
+            else fields.Profile("ring", P0, width=w, radius=r0))
+    spec = {"grid": fields.Grid("fraction", -L / 2.0, L), "pump": pump,
+            "state": fields.Profile("gaussian", 0.1, width=math.sqrt(2.0)), "noise": 1e-3,
+            "seed": int(rng.integers(0, 2 ** 62))}
+    return Request(rid, arrival, dim, n, steps, stable_dt(dim, h), h, configs.C_MODEL.copy(), spec=spec)
+
+
+def poisson_requests(rate, duration, seed=0, step_range=(1e3, 1e5), device_inputs=False):
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     while True:
         t += rng.exponential(1.0 / rate)
         if t > duration:
             break
-        out.append(make_request(len(out), t, rng, step_range))
+        out.append(make_request(len(out), t, rng, step_range, device_inputs=device_inputs))
     return out
 
 
