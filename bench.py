@@ -93,9 +93,14 @@ def single_grid_2048(dev, steps=100):
     path, and the compulsory 2 S + r (SSF: 4 c + 3 r)."""
     import torch
     from nls_amd import configs, native
+    from nls_amd.autotune import Tuner
     n = 2048
     out = {}
+    tuner = Tuner(path=None, steps=32, reps=2)
     for dtype in ("f64", "f32"):
+        rows = tuner.stage_tile(dtype, n=n)          # tile height of the per-stage kernels, measured here
+        log("2048^2 %s: per-stage tile height %d (%s)" % (dtype, rows, ", ".join(
+            "%s rows %.1f us" % (k, v * 1e6) for k, v in tuner.last.items())))
         cs, rs = (16, 8) if dtype == "f64" else (8, 4)
         S = cs + rs
         rt = torch.float64 if dtype == "f64" else torch.float32
@@ -121,6 +126,8 @@ def single_grid_2048(dev, steps=100):
             if kbytes:
                 r["hbm_frac_kstorage_bytes"] = rate * kbytes / HBM_PEAK
                 r["vs_kstorage_ceiling"] = rate / (HBM_PEAK / kbytes)
+            if name == "rk4_stage":
+                r["tile_rows"] = rows
             out["%s_%s" % (name, dtype)] = r
             log("2048^2 %s %s: %.1f us/step, %.2f G grid-point-steps/s" % (name, dtype, t / steps * 1e6, rate / 1e9))
     return out

@@ -146,7 +146,7 @@ int nls_release(void);
 int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const void *nres,
             void *dpsi, void *dn, void *stream);
 
-/* Tile height (rows: 8, 16 or 32) of the single-instance 2-D RK4 stage kernels
+/* Tile height (rows: 4, 8, 16 or 32) of the single-instance 2-D RK4 stage kernels
  * for dtype; rows = 0 only queries.  Returns the height in effect or a
  * negative NLS_E* code.  Process-wide; set by the autotuner.  No
  * counterpart in the reference. */

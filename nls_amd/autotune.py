@@ -10,7 +10,7 @@ What is tuned:
   * ``plain_engine(desc)``   -- engine of a plain solve (nls_advance): resident
     (instances that fit on chip), fused (2-D) or per-stage, each with its
     step-block graphs;
-  * ``stage_tile(dtype)``    -- tile height (8, 16, 32 rows) of the 2-D
+  * ``stage_tile(dtype)``    -- tile height (4, 8, 16, 32 rows) of the 2-D
     per-stage kernels (nls_tune_stage_tile), applied to the library;
   * ``batch_engine(members, ...)`` -- engine of a batch (nls_batch_create):
     resident, fused or per-stage for that member mix.
@@ -27,7 +27,7 @@ from . import native
 
 ENGINES_2D = ("resident", "fused", "stage")
 ENGINES_1D = ("resident", "stage")
-STAGE_TILES = (8, 16, 32)
+STAGE_TILES = (4, 8, 16, 32)
 
 
 def default_cache_path():

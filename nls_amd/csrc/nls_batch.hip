@@ -277,9 +277,20 @@ k_fused_step_2d(const FusedArgs<T> f) {
         if (!DB) ynext[j] = mk<T>(0, 0);
         li[j] = 0;
         if (NP % NT == 0 || j < J - 1 || k < NP) {
-            const unsigned e = map[k];
-            li[j] = e & 0xffff;
-            int gx = td.x0 - H + (int)((e >> 16) & 255), gy = td.y0 - H + (int)(e >> 24);
+            // the first TX*TY list entries are the output tile, row-major:
+            // their coordinates need no table load
+            int lx, ly;
+            if (j < JOUT) {
+                lx = H + (k & (TX - 1));
+                ly = H + k / TX;
+                li[j] = ly * WS + lx;
+            } else {
+                const unsigned e = map[k];
+                li[j] = e & 0xffff;
+                lx = (int)((e >> 16) & 255);
+                ly = (int)(e >> 24);
+            }
+            int gx = td.x0 - H + lx, gy = td.y0 - H + ly;
             bool ok = true;
             if (BC == BC_PERIODIC) {
                 if (near) { gx = wrap_near(gx, n); gy = wrap_near(gy, n); }

@@ -219,6 +219,7 @@ constexpr size_t GRAPH_CACHE = 8;
 
 struct GraphEntry {
     int device;
+    int stage_ty;          // tile height of the per-stage kernels when captured
     nls_problem p;
     const void *P;
     void *psi, *nres, *ws;
@@ -248,8 +249,11 @@ static int block_graph(const nls_problem *p, const void *P, void *psi, void *nre
     if (rc) return rc;
     if (!g_graphs) g_graphs = new GraphCache;
     GraphCache &gc = *g_graphs;
+    // a graph holds the kernels of the launch configuration it was captured
+    // with: the per-stage tile height (nls_tune_stage_tile) is part of the key
+    const int ty = get_stage_tile(p->dtype);
     for (GraphEntry &e : gc.entries)
-        if (e.device == dev && e.P == P && e.psi == psi && e.nres == nres && e.ws == ws &&
+        if (e.device == dev && e.stage_ty == ty && e.P == P && e.psi == psi && e.nres == nres && e.ws == ws &&
             memcmp(&e.p, p, sizeof *p) == 0) {
             e.used = ++gc.clock;
             *out = e.exec;
@@ -287,6 +291,7 @@ static int block_graph(const nls_problem *p, const void *P, void *psi, void *nre
     }
     GraphEntry e;
     e.device = dev;
+    e.stage_ty = ty;
     e.p = *p;
     e.P = P; e.psi = psi; e.nres = nres; e.ws = ws;
     e.exec = exec;
@@ -419,7 +424,7 @@ int nls_solve_nls_2d(float dt, float dx, int n, int order, int iters, const floa
 int nls_tune_stage_tile(int dtype, int rows) {
     if (dtype != NLS_F32 && dtype != NLS_F64) return fail(NLS_EINVAL, "bad dtype");
     if (rows == 0) return get_stage_tile(dtype);
-    if (set_stage_tile(dtype, rows)) return fail(NLS_EINVAL, "tile height must be 8, 16 or 32 (got %d)", rows);
+    if (set_stage_tile(dtype, rows)) return fail(NLS_EINVAL, "tile height must be 4, 8, 16 or 32 (got %d)", rows);
     return rows;
 }
 

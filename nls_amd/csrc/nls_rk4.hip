@@ -3,6 +3,8 @@
 // float or double, for the reference's boundary families (1-D radial / 2-D
 // zero-outside) and periodic grids, with the algebraic or the dynamic
 // reservoir.  2-D: one 64 x 16 tile per workgroup; 1-D: 256 points.
+#include <atomic>
+
 #include "nls_rk4.h"
 
 namespace nls {
@@ -37,7 +39,9 @@ __global__ void k_radial_table(int n, int order, double h, T *wtab) {
     for (int k = 0; k < order; ++k) wtab[(size_t)i * order + k] = (T)radial_weight(i, k, n, order, h);
 }
 
-static int g_stage_ty[2] = {16, 16};      // tile height of the 2-D stage kernels per dtype
+// tile height of the 2-D stage kernels per dtype (a process-wide knob, set
+// by nls_tune_stage_tile from any thread)
+static std::atomic<int> g_stage_ty[2] = {{16}, {16}};
 
 template <typename T, int ORDER, int BC, bool DYN>
 static void launch_stage(int dim, const StageArgs<T> &a, hipStream_t s) {
@@ -46,7 +50,10 @@ static void launch_stage(int dim, const StageArgs<T> &a, hipStream_t s) {
         const int ntx = (a.n + TILE2_X - 1) / TILE2_X;
         const int nty = (a.n + ty - 1) / ty;
         const int ntiles = ntx * nty, chunk = (ntiles + 7) / 8;
-        if (ty == 8)
+        if (ty == 4)
+            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 4>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
+                               ntiles, chunk);
+        else if (ty == 8)
             hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 8>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
                                ntiles, chunk);
         else if (ty == 16)
@@ -72,7 +79,7 @@ void launch_rk4_stage(int dim, int order, int bc, bool dyn, const StageArgs<T> &
 }
 
 int set_stage_tile(int dtype, int ty) {
-    if (ty != 8 && ty != 16 && ty != 32) return -1;
+    if (ty != 4 && ty != 8 && ty != 16 && ty != 32) return -1;
     g_stage_ty[dtype ? 1 : 0] = ty;
     return 0;
 }

@@ -48,7 +48,7 @@ def test_fastest_candidate_is_chosen_and_cached(tmp_path):
 
 def test_stage_tile_and_batch_choices():
     calls = []
-    T = autotune.Tuner(path=None, timer=lambda cand, _s: {8: 3.0, 16: 1.0, 32: 2.0}[cand[1]], device_name="dev")
+    T = autotune.Tuner(path=None, timer=lambda cand, _s: {4: 4.0, 8: 3.0, 16: 1.0, 32: 2.0}[cand[1]], device_name="dev")
     assert T.stage_tile("f32") == 16
     ms = configs.sweep512(count=4, steps=10)
     T = autotune.Tuner(path=None, timer=fake_timer({"resident": 9, "fused": 1, "stage": 2}, calls), device_name="dev")
@@ -74,5 +74,5 @@ def test_tuned_plain_solve_matches_oracle(gpu, oracle, tmp_path):
     opsi, _ = oracle.rk4(2, n, 5, 1, 1e-3, h, 200, P, c, psi0, n0)
     assert np.linalg.norm(psi - opsi) / np.linalg.norm(opsi) <= 1e-10
     rows = T.stage_tile("f64", n=256)
-    assert rows in (8, 16, 32) and native.stage_tile("f64") == rows
+    assert rows in (4, 8, 16, 32) and native.stage_tile("f64") == rows
     native.stage_tile("f64", 16)

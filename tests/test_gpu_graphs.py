@@ -101,3 +101,25 @@ def test_graph_cache_eviction(gpu):
         torch.cuda.synchronize()
         np.testing.assert_array_equal(psi.cpu().numpy(), ref)
     native.release()
+
+
+def test_graph_follows_stage_tile_height(gpu):
+    """The cached graph of a per-stage solve is keyed by the tile height: after
+    nls_tune_stage_tile the next solve runs kernels of the new height (same
+    results, since the tile shape does not change a point's arithmetic)."""
+    import torch
+    d_graph, d_eager, Pt, state, ws, ws2, _ = setup(2, 96, "f64", "stage")
+    old = native.stage_tile("f64")
+    outs = []
+    try:
+        for rows in (8, 32, 16):
+            native.stage_tile("f64", rows)
+            psi, nres = state()
+            native.advance(d_graph, 130, Pt, psi, nres, ws)
+            torch.cuda.synchronize()
+            outs.append(psi.cpu().numpy())
+    finally:
+        native.stage_tile("f64", old)
+    np.testing.assert_array_equal(outs[0], outs[1])
+    np.testing.assert_array_equal(outs[0], outs[2])
+    native.release()
