@@ -133,6 +133,38 @@ def single_grid_2048(dev, steps=100):
     return out
 
 
+def config_accuracy(dev):
+    """Relative L2 error of the FP32 (complex64) solve against the FP64 solve
+    after the full step count of each configuration: line256 (1-D N=256,
+    2000 steps), square256 (256^2, 10^4 steps), square2048 (2048^2, 2*10^4
+    steps, RK4 and split-step Fourier).  The FP64 GPU path is the reference
+    here; the tests pin it to the FP64 oracle (relL2 <= 1e-10)."""
+    import numpy as np
+    import torch
+    from nls_amd import configs, native
+    out = {}
+    cases = [("line256", configs.line256(), "rk4"), ("square256", configs.square256(), "rk4"),
+             ("square2048_rk4", configs.square2048(), "rk4"), ("square2048_ssf", configs.square2048(), "ssf")]
+    for name, inst, scheme in cases:
+        res = {}
+        for dtype in ("f64", "f32"):
+            d = native.ProblemDesc.make(inst.dim, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic",
+                                        dtype=dtype, scheme=scheme)
+            rt = torch.float64 if dtype == "f64" else torch.float32
+            ct = torch.complex128 if dtype == "f64" else torch.complex64
+            P = torch.as_tensor(np.ascontiguousarray(inst.P), dtype=rt, device=dev)
+            psi = torch.as_tensor(np.ascontiguousarray(inst.psi0), dtype=ct, device=dev)
+            nres = torch.as_tensor(np.ascontiguousarray(inst.n0), dtype=rt, device=dev)
+            ws = torch.empty(max(1, native.workspace_bytes(d)), dtype=torch.uint8, device=dev)
+            native.advance(d, inst.steps, P, psi, nres, ws)
+            torch.cuda.synchronize()
+            res[dtype] = psi.cpu().numpy().astype(np.complex128)
+        err = float(np.linalg.norm(res["f32"] - res["f64"]) / np.linalg.norm(res["f64"]))
+        out[name] = {"relL2_f32_vs_f64": err, "steps": inst.steps, "points": inst.points}
+        log("accuracy %s (%d steps): FP32 vs FP64 relL2 = %.2e" % (name, inst.steps, err))
+    return out
+
+
 def serving(rate, window, seed=0):
     """Continuous batching: Poisson arrivals at `rate` per second over `window`
     seconds, the request mix of nls_amd.serving.make_request with inputs given
@@ -161,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison and the 2048^2 grid")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop latency measurement")
+    ap.add_argument("--no-accuracy", action="store_true",
+                    help="skip the FP32-vs-FP64 accuracy of the line256/square256/square2048 configurations")
     ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
     ap.add_argument("--serve-window", type=float, default=2.0, help="arrival window (s)")
     ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "stage", "resident", "auto"),
@@ -296,6 +330,8 @@ def main():
         out["single_grid_2048"] = single_grid_2048(dev)
 
     batch.close()
+    if not args.no_accuracy:
+        out["config_accuracy"] = config_accuracy(dev)
     if not args.no_serve:
         out["serving"] = serving(args.serve_rate, args.serve_window)
     if not args.no_cpu:

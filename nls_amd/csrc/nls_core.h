@@ -74,6 +74,34 @@ __device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, 
     rhs_point_cp<T, DYN>(c, pump_term<T, DYN>(c, P), y, yn, lap, k, kn);
 }
 
+// Cartesian Laplacian of a point with value c from its neighbours at
+// distance d = 1..R (weights w_d of nls_stencil.h).
+//   double precision: the stencil sum as written, the centre weight times c
+//     (lap_centre) plus w_d times the sum of the neighbours at distance d;
+//   single precision: the centre weight is folded into the neighbour terms
+//     (the weights of a stencil sum to zero).  For d = 1 the differences to c
+//     are formed first -- exact when the values are close, as on a smooth
+//     field on a fine grid -- so the largest weights (w_1 ~ 1/h^2) no longer
+//     amplify the rounding of c; for d >= 2 the neighbour sum less 2c (1-D) or
+//     4c (2-D).  This keeps the single-precision Laplacian accurate where the
+//     stencil sum cancels to a small result.
+// The two forms are equal up to rounding.
+template <typename T> __device__ __forceinline__ T lap_centre(T wc, T c) {
+    return sizeof(T) == 8 ? wc * c : (T)0;
+}
+
+template <typename T> __device__ __forceinline__ T lap_pair(T w, T c, T a0, T a1, int d) {
+    if (sizeof(T) == 8) return w * (a0 + a1);
+    if (d == 1) return w * ((a0 - c) + (a1 - c));
+    return w * ((a0 + a1) - (T)2 * c);
+}
+
+template <typename T> __device__ __forceinline__ T lap_quad(T w, T c, T a0, T a1, T b0, T b1, int d) {
+    if (sizeof(T) == 8) return w * ((a0 + a1) + (b0 + b1));
+    if (d == 1) return w * (((a0 - c) + (a1 - c)) + ((b0 - c) + (b1 - c)));
+    return w * (((a0 + a1) + (b0 + b1)) - (T)4 * c);
+}
+
 // Periodic index wrap for offsets that may exceed one period.
 __device__ __forceinline__ int wrap_index(int g, int n) {
     while (g < 0) g += n;

@@ -190,13 +190,13 @@ __device__ __forceinline__ void stage_tile_2d(const StageArgs<T> &a, int x0, int
         if (!own[j]) continue;
         const int l = (ty + 4 * j + R) * LW + tx + R;
         const C c = tile[l];
-        T lr = (T)2 * w[0] * c.x, li = (T)2 * w[0] * c.y;
+        T lr = lap_centre<T>((T)2 * w[0], c.x), li = lap_centre<T>((T)2 * w[0], c.y);
 #pragma unroll
         for (int d = 1; d <= R; ++d) {
             const C a0 = tile[l - d], a1 = tile[l + d];
             const C b0 = tile[l - d * LW], b1 = tile[l + d * LW];
-            lr += w[d] * ((a0.x + a1.x) + (b0.x + b1.x));
-            li += w[d] * ((a0.y + a1.y) + (b0.y + b1.y));
+            lr += lap_quad<T>(w[d], c.x, a0.x, a1.x, b0.x, b1.x, d);
+            li += lap_quad<T>(w[d], c.y, a0.y, a1.y, b0.y, b1.y, d);
         }
         T yn = 0;
         if (DYN) yn = first ? un[j] : un[j] + (kn_in[j] * a.dt) * a.scale;
@@ -232,13 +232,13 @@ __device__ __forceinline__ void stage_tile_1d(const StageArgs<T> &a, int x0, cpl
     T lr = 0, li = 0;
     if (BC == BC_PERIODIC) {
         const T w[4] = {a.w0, a.w1, a.w2, a.w3};
-        lr = w[0] * c.x;
-        li = w[0] * c.y;
+        lr = lap_centre<T>(w[0], c.x);
+        li = lap_centre<T>(w[0], c.y);
 #pragma unroll
         for (int d = 1; d <= R; ++d) {
             const cplx_t<T> a0 = tile[l - d], a1 = tile[l + d];
-            lr += w[d] * (a0.x + a1.x);
-            li += w[d] * (a0.y + a1.y);
+            lr += lap_pair<T>(w[d], c.x, a0.x, a1.x, d);
+            li += lap_pair<T>(w[d], c.y, a0.y, a1.y, d);
         }
     } else {
         const T *wr = a.wtab + (size_t)gx * ORDER;
