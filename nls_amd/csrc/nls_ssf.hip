@@ -330,19 +330,43 @@ __global__ void __launch_bounds__(256) k_ssf_line(const SsfArgs<T> a) {
     }
 }
 
-// Twiddles and phase, double precision, rounded once to T.
+// The unit complex number (c, s) in T.  In single precision the rounded
+// pair has |.|^2 = 1 + O(6e-8), a fixed gain or loss of the mode it
+// multiplies on every use; of the pairs within one ulp of (c, s) the one
+// closest to the unit circle is taken (|.|^2 - 1 ~ 1e-9), so thousands of
+// transforms and phase steps do not drift the mode amplitudes.
+template <typename T> __device__ __forceinline__ cplx_t<T> unit_round(double c, double s) {
+    if (sizeof(T) == 8) return mk<T>((T)c, (T)s);
+    const float c0 = (float)c, s0 = (float)s;
+    float bc = c0, bs = s0;
+    double best = fabs((double)c0 * c0 + (double)s0 * s0 - 1.0);
+    const float cs[3] = {nextafterf(c0, -2.0f), c0, nextafterf(c0, 2.0f)};
+    const float ss[3] = {nextafterf(s0, -2.0f), s0, nextafterf(s0, 2.0f)};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double e = fabs((double)cs[i] * cs[i] + (double)ss[j] * ss[j] - 1.0);
+            if (e < best) { best = e; bc = cs[i]; bs = ss[j]; }
+        }
+    return mk<T>((T)bc, (T)bs);
+}
+
+// Twiddles and phase, double precision, rounded to T on the unit circle
+// (unit_round); the phase carries the transform's 1/N (a power of two).
 template <typename T>
 __global__ void k_ssf_tables(int N, double dx, double dt, double scale, cplx_t<T> *tw, cplx_t<T> *ph) {
     const int m = blockIdx.x * blockDim.x + threadIdx.x;
     if (m >= N) return;
     double s, c;
     sincospi(2.0 * (double)m / (double)N, &s, &c);
-    tw[m] = mk<T>((T)c, (T)-s);
+    tw[m] = unit_round<T>(c, -s);
     const int f = m < N / 2 ? m : m - N;                 // FFT frequency order
     const double k = 2.0 * M_PI * (double)f / ((double)N * dx);
     double ps, pc;
     sincos(k * k * dt, &ps, &pc);
-    ph[m] = mk<T>((T)(pc * scale), (T)(-ps * scale));
+    const cplx_t<T> u = unit_round<T>(pc, -ps);
+    ph[m] = mk<T>(u.x * (T)scale, u.y * (T)scale);
 }
 
 static size_t align256s(size_t b) { return (b + 255) & ~(size_t)255; }
