@@ -195,7 +195,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop latency measurement")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-vs-FP64 accuracy of the line256/square256/square2048 configurations")
-    ap.add_argument("--serve-rate", type=float, default=50.0, help="Poisson arrival rate (requests/s)")
+    ap.add_argument("--serve-rates", default="20,50,100,200,500",
+                    help="Poisson arrival rates (requests/s, comma-separated), one request-loop run each")
     ap.add_argument("--serve-window", type=float, default=2.0, help="arrival window (s)")
     ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "stage", "resident", "auto"),
                     help="batch engine of the headline workload (tuned: measured on this device)")
@@ -333,7 +334,7 @@ def main():
     if not args.no_accuracy:
         out["config_accuracy"] = config_accuracy(dev)
     if not args.no_serve:
-        out["serving"] = serving(args.serve_rate, args.serve_window)
+        out["serving"] = [serving(float(r), args.serve_window) for r in args.serve_rates.split(",") if r]
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(members[:16], 200)
         log("cpu baseline: %.3g grid-point-steps/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["kind"]))
