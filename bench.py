@@ -133,6 +133,80 @@ def single_grid_2048(dev, steps=100):
     return out
 
 
+XGMI_LINK = 153e9     # bytes/s of one MI355X xGMI link
+
+
+def slab_shards(dev, steps=20, reps=3):
+    """Row-slab decomposition at its shard shape (nls_slab_step): one of 8
+    row slabs of a 2048^2 and of a 16384^2 periodic grid (order 5, dynamic
+    reservoir), with its 8 ghost rows per side, against 1/8 of the whole-grid
+    fused step on this device.  Slab steps are captured in a graph (one
+    launch per step, as a device would issue them); the ghost rows a slab
+    receives per step and their time over one xGMI link per neighbour are
+    estimates (one device here)."""
+    import torch
+    from nls_amd import configs, native
+    out = {}
+    h = native.slab_halo_rows(5)
+    for n in (2048, 16384):
+        for dtype in ("f32", "f64"):
+            cs, rs = (16, 8) if dtype == "f64" else (8, 4)
+            rt = torch.float64 if dtype == "f64" else torch.float32
+            ct = torch.complex128 if dtype == "f64" else torch.complex64
+            d = native.ProblemDesc.make(2, n, 5, 1e-4, 0.0625, configs.model_coeffs(dynamic=True), bc="periodic",
+                                        dtype=dtype, engine="fused", graphs=False)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            # whole grid
+            P = torch.ones((n, n), dtype=rt, device=dev)
+            psi = torch.full((n, n), 0.1, dtype=ct, device=dev)
+            nres = torch.zeros((n, n), dtype=rt, device=dev)
+            ws = torch.empty(max(1, native.workspace_bytes(d)), dtype=torch.uint8, device=dev)
+            native.advance(d, 2, P, psi, nres, ws)
+            torch.cuda.synchronize()
+            e0.record()
+            native.advance(d, steps, P, psi, nres, ws)
+            e1.record()
+            torch.cuda.synchronize()
+            whole_us = e0.elapsed_time(e1) * 1e3 / steps
+            del P, psi, nres, ws
+            torch.cuda.empty_cache()
+            # one slab of n/8 rows
+            rows = n // 8
+            Ps = torch.ones((rows + 2 * h, n), dtype=rt, device=dev)
+            a = torch.full((rows + 2 * h, n), 0.1, dtype=ct, device=dev)
+            b = torch.full_like(a, 0.1)
+            na, nb = torch.zeros((rows + 2 * h, n), dtype=rt, device=dev), torch.zeros((rows + 2 * h, n), dtype=rt,
+                                                                                        device=dev)
+            native.slab_step(d, rows, Ps, a, na, b, nb)
+            torch.cuda.synchronize()
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                for i in range(steps):
+                    src, dst = ((a, na), (b, nb)) if i % 2 == 0 else ((b, nb), (a, na))
+                    native.slab_step(d, rows, Ps, src[0], src[1], dst[0], dst[1])
+            g.replay()
+            torch.cuda.synchronize()
+            e0.record()
+            for _ in range(reps):
+                g.replay()
+            e1.record()
+            torch.cuda.synchronize()
+            slab_us = e0.elapsed_time(e1) * 1e3 / (reps * steps)
+            del g, Ps, a, b, na, nb
+            torch.cuda.empty_cache()
+            ghost = h * n * (cs + rs)                 # bytes from each neighbour per step
+            key = "%dx%d_%s" % (n, rows, dtype)
+            out[key] = {"slab_us_per_step": slab_us, "whole_grid_us_per_step": whole_us,
+                        "whole_over_8_us": whole_us / 8, "slab_vs_whole_over_8": slab_us / (whole_us / 8),
+                        "ghost_rows_per_side": h, "ghost_bytes_per_neighbour_per_step": ghost,
+                        "xgmi_us_per_neighbour_estimate": ghost / XGMI_LINK * 1e6}
+            log("slab %s: %.1f us/step vs whole grid %.1f us / 8 = %.1f us (x%.2f); ghost rows %.2f MB per "
+                "neighbour per step, ~%.1f us over one xGMI link" % (key, slab_us, whole_us, whole_us / 8,
+                                                                     slab_us / (whole_us / 8), ghost / 1e6,
+                                                                     ghost / XGMI_LINK * 1e6))
+    return out
+
+
 def config_accuracy(dev):
     """Relative L2 error of the FP32 (complex64) solve against the FP64 solve
     after the full step count of each configuration: line256 (1-D N=256,
@@ -191,7 +265,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--no-plain", action="store_true", help="skip the plain FP64 comparison and the 2048^2 grid")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="skip the plain FP64 comparison, the 2048^2 grid and the row-slab shards")
     ap.add_argument("--no-serve", action="store_true", help="skip the request-loop latency measurement")
     ap.add_argument("--no-accuracy", action="store_true",
                     help="skip the FP32-vs-FP64 accuracy of the line256/square256/square2048 configurations")
@@ -202,6 +277,8 @@ def main():
                     help="batch engine of the headline workload (tuned: measured on this device)")
     ap.add_argument("--grid2048-only", action="store_true",
                     help="only measure the 2048 x 2048 single grid (plain path) and print its JSON")
+    ap.add_argument("--slab-only", action="store_true",
+                    help="only measure the row-slab shards (one of 8 slabs of 2048^2 and 16384^2) and print its JSON")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run 5 device-to-device copies of BYTES (calibrates rocprofv3 byte counters)")
     args = ap.parse_args()
@@ -229,6 +306,9 @@ def main():
 
     if args.grid2048_only:
         print(json.dumps({"single_grid_2048": single_grid_2048(dev)}))
+        return
+    if args.slab_only:
+        print(json.dumps({"slab_shards": slab_shards(dev)}))
         return
 
     t0 = time.perf_counter()
@@ -329,6 +409,7 @@ def main():
         out["accuracy"] = {"max_relL2_vs_f64": max(errs.values()), "per_size": {str(k): v for k, v in errs.items()},
                            "tolerance": 1e-4}
         out["single_grid_2048"] = single_grid_2048(dev)
+        out["slab_shards"] = slab_shards(dev)
 
     batch.close()
     if not args.no_accuracy:

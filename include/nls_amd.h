@@ -140,6 +140,23 @@ int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi,
  * the device).  Optional: the cache is bounded. */
 int nls_release(void);
 
+/* Row-slab decomposition of one periodic 2-D grid (one device's share of a
+ * grid split along its rows; no counterpart in the reference, whose solvers
+ * are serial).  A slab of `rows` rows of an n-wide grid (p->n = n) is held in
+ * buffers of (rows + 2 h) x n points, h = nls_slab_halo_rows(p->order) = 4R
+ * = 2 (order - 1): h ghost rows (the h rows above the slab, periodically),
+ * the slab's rows, h ghost rows (the rows below it).  nls_slab_step runs one
+ * RK4 step on the fused kernel and writes rows h .. h + rows - 1 of psi_out
+ * and nres_out (ghost rows untouched); psi_in / nres_in / pumping are read
+ * in full.  The 4R-deep halo is what one step depends on, so one exchange
+ * of h rows with each neighbour per step keeps the slabs equal, bit for bit,
+ * to the same rows of the whole-grid solve on the fused engine.  Needs
+ * dim 2, periodic bc, the RK4 scheme and rows >= h; in and out buffers must
+ * not overlap.  Enqueues on `stream` and returns. */
+int nls_slab_halo_rows(int order);
+int nls_slab_step(const nls_problem *p, int rows, const void *pumping, const void *psi_in,
+                  const void *nres_in, void *psi_out, void *nres_out, void *stream);
+
 /* One evaluation of the right-hand side (hamiltonian / hamiltonian_2d,
  * nls.f90:621-650, :841-870) on device arrays: dpsi = F_psi, dn = F_n
  * (dn only for the dynamic reservoir). */

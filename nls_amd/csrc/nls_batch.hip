@@ -145,6 +145,9 @@ template <typename T> struct FusedArgs {
     // grid described by `grid`; tile t sits at (t % grid_ntx, t / grid_ntx).
     int grid_ntx;
     MemberDev<T> grid;
+    // BC_SLAB: buffer rows 0..4R-1 and slab_end..slab_end+4R-1 are ghost
+    // rows; rows 4R..slab_end-1 are the slab's own (the only ones written)
+    int slab_end;
 };
 
 // Tile shape and launch shape of the fused kernel.  TY: output rows.  NT:
@@ -224,13 +227,14 @@ k_fused_step_2d(const FusedArgs<T> f) {
     TileDesc td;
     MemberDev<T> m;
     if (f.grid_ntx > 0) {
-        td.m = 0; td.x0 = (t % f.grid_ntx) * TX; td.y0 = (t / f.grid_ntx) * TY;
+        td.m = 0; td.x0 = (t % f.grid_ntx) * TX; td.y0 = (t / f.grid_ntx) * TY + (BC == BC_SLAB ? H : 0);
         m = f.grid;
     } else {
         td = f.tiles[t];
         m = f.mem[td.m];
     }
     const int n = m.n;
+    const int ylim = BC == BC_SLAB ? f.slab_end : n;   // output rows end here
     const long long off = m.off;
     const C *u = f.u + off;
     const T *un = DYN ? f.un + off : nullptr;
@@ -244,7 +248,7 @@ k_fused_step_2d(const FusedArgs<T> f) {
         for (int j = 0; j < JOUT; ++j) {
             const int k = tid + NT * j;
             const int gx = td.x0 + (k & (TX - 1)), gy = td.y0 + k / TX;
-            if (gx >= n || gy >= n) continue;
+            if (gx >= n || gy >= ylim) continue;
             const size_t gi = (size_t)gy * n + gx;
             uo[gi] = u[gi];
             if (DYN) uno[gi] = un[gi];
@@ -295,6 +299,11 @@ k_fused_step_2d(const FusedArgs<T> f) {
             if (BC == BC_PERIODIC) {
                 if (near) { gx = wrap_near(gx, n); gy = wrap_near(gy, n); }
                 else { gx = wrap_index(gx, n); gy = wrap_index(gy, n); }
+            } else if (BC == BC_SLAB) {
+                // rows from 0 (the tile rows start at 4R); rows past the
+                // bottom ghost rows reach no written point within 4R rows
+                gx = near ? wrap_near(gx, n) : wrap_index(gx, n);
+                ok = gy < ylim + H;
             } else {
                 ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
             }
@@ -384,7 +393,7 @@ k_fused_step_2d(const FusedArgs<T> f) {
     for (int j = 0; j < JOUT; ++j) {
         const int k0 = tid + NT * j;
         const int gx = td.x0 + (k0 & (TX - 1)), gy = td.y0 + k0 / TX;
-        if (gx >= n || gy >= n) continue;
+        if (gx >= n || gy >= ylim) continue;
         C k;
         T kn;
         stage_rhs(yb, j, k, kn);
@@ -585,6 +594,8 @@ static void launch_fused(int order, int bc, bool dyn, const FusedArgs<T> &f, hip
     NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
     NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
     NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
+    NLS_CASE(3, 2, false) NLS_CASE(3, 2, true) NLS_CASE(5, 2, false) NLS_CASE(5, 2, true)
+    NLS_CASE(7, 2, false) NLS_CASE(7, 2, true)
 #undef NLS_CASE
 }
 
@@ -677,6 +688,38 @@ static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T>
         if (dyn) rc = hip_check(hipMemcpyAsync(nres, curn, N * sizeof(T), hipMemcpyDeviceToDevice, s), "copy reservoir");
     }
     return rc;
+}
+
+// One RK4 step of one row slab of a grid periodic along its rows (BC_SLAB):
+// buffers of (rows + 8R) x n points, the slab's rows preceded and followed by
+// 4R ghost rows; the step writes rows 4R .. 4R + rows - 1 of the output.
+// The 4R-deep halo is exactly what the fused kernel recomputes on chip, so
+// one exchange of ghost rows per step suffices.
+template <typename T>
+static int fused_slab_run(const nls_problem *p, int rows, const T *P, const cplx_t<T> *psi_in, const T *nres_in,
+                          cplx_t<T> *psi_out, T *nres_out, hipStream_t s) {
+    const int H = 2 * (p->order - 1);
+    const int ty = fused_ty(p->dtype, p->order);
+    FusedArgs<T> f;
+    memset(&f, 0, sizeof f);
+    f.grid_ntx = (p->n + 63) / 64;
+    f.ntiles = f.grid_ntx * ((rows + ty - 1) / ty);
+    f.chunk = (f.ntiles + 7) / 8;
+    f.grid = plain_member<T>(p);
+    f.slab_end = H + rows;
+    f.P = P;
+    f.u = psi_in; f.un = nres_in; f.uo = psi_out; f.uno = nres_out;
+    launch_fused<T>(p->order, BC_SLAB, p->coeffs[10] != 0.0, f, s);
+    return hip_check(hipGetLastError(), "slab step launch");
+}
+
+int fused_slab_step(const nls_problem *p, int rows, const void *P, const void *psi_in, const void *nres_in,
+                    void *psi_out, void *nres_out, hipStream_t s) {
+    if (p->dtype == NLS_F64)
+        return fused_slab_run<double>(p, rows, (const double *)P, (const cplx_t<double> *)psi_in,
+                                      (const double *)nres_in, (cplx_t<double> *)psi_out, (double *)nres_out, s);
+    return fused_slab_run<float>(p, rows, (const float *)P, (const cplx_t<float> *)psi_in, (const float *)nres_in,
+                                 (cplx_t<float> *)psi_out, (float *)nres_out, s);
 }
 
 // Plain solve of one line (n <= 4096) or small grid (n <= 64) on the

@@ -336,6 +336,30 @@ int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi,
     return rest ? advance_direct(p, rest, pumping, psi, nres, workspace, s) : NLS_OK;
 }
 
+int nls_slab_halo_rows(int order) {
+    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7 (got %d)", order);
+    return 2 * (order - 1);
+}
+
+int nls_slab_step(const nls_problem *p, int rows, const void *pumping, const void *psi_in, const void *nres_in,
+                  void *psi_out, void *nres_out, void *stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (p->dim != 2 || p->bc != NLS_BC_PERIODIC || p->scheme != NLS_SCHEME_RK4)
+        return fail(NLS_EUNSUPPORTED, "row slabs are 2-D, periodic and RK4");
+    const int h = 2 * (p->order - 1);
+    if (rows < h) return fail(NLS_EINVAL, "rows=%d is smaller than the %d ghost rows of a side", rows, h);
+    if ((long long)(rows + 2 * h) * p->n >= (1ll << 31))
+        return fail(NLS_EINVAL, "slab of %d rows x %d points too large", rows, p->n);
+    const bool dyn = p->coeffs[10] != 0.0;
+    if (!pumping || !psi_in || !psi_out || (dyn && (!nres_in || !nres_out)))
+        return fail(NLS_EINVAL, "NULL device buffer");
+    if (psi_in == psi_out || (dyn && nres_in == nres_out))
+        return fail(NLS_EINVAL, "input and output slabs must be different buffers");
+    return fused_slab_step(p, rows, pumping, psi_in, dyn ? nres_in : nullptr, psi_out, dyn ? nres_out : nullptr,
+                           (hipStream_t)stream);
+}
+
 int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const void *nres, void *dpsi, void *dn,
             void *stream) {
     int rc = check_problem(p);

@@ -9,7 +9,10 @@
 
 namespace nls {
 
-enum { BC_REF = 0, BC_PERIODIC = 1 };
+// BC_SLAB (2-D fused kernel only): one row slab of a grid that is periodic
+// along its rows, with the 4R ghost rows above and below supplied in the
+// buffers; x wraps, y reads the ghost rows.
+enum { BC_REF = 0, BC_PERIODIC = 1, BC_SLAB = 2 };
 
 template <typename T> struct cvec;
 template <> struct cvec<float> { using type = float2; };

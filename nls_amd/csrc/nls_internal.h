@@ -18,6 +18,9 @@ int plain_engine(const nls_problem *p);            // RK4 engine of a plain solv
 size_t fused_grid_workspace_bytes(const nls_problem *p);
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
                        hipStream_t s);
+// one RK4 step of a row slab with 4R ghost rows on each side (fused kernel, BC_SLAB)
+int fused_slab_step(const nls_problem *p, int rows, const void *P, const void *psi_in, const void *nres_in,
+                    void *psi_out, void *nres_out, hipStream_t s);
 // plain RK4 solve of one line (n <= 4096) or small grid (n <= 64) on the resident kernel
 bool resident_fits(const nls_problem *p);
 size_t resident_workspace_bytes(const nls_problem *p);

@@ -33,7 +33,7 @@ EXPORTS = (
     "nls_chemical_potential", "nls_observables",
     "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
     "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance", "nls_tune_stage_tile",
-    "nls_release", "nls_fill_profile", "nls_fill_state",
+    "nls_release", "nls_fill_profile", "nls_fill_state", "nls_slab_halo_rows", "nls_slab_step",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2}
 FLAG_EAGER = 1
@@ -135,6 +135,8 @@ def load():
         "nls_workspace_bytes": ([pp], sz),
         "nls_advance": ([pp, i, vp, vp, vp, vp, sz, vp], i),
         "nls_release": ([], i),
+        "nls_slab_halo_rows": ([i], i),
+        "nls_slab_step": ([pp, i, vp, vp, vp, vp, vp, vp], i),
         "nls_fill_profile": ([C.POINTER(ProfileDesc), i, i, C.POINTER(GridDesc), i, vp, vp], i),
         "nls_fill_state": ([C.POINTER(ProfileDesc), d, C.c_uint64, i, i, C.POINTER(GridDesc), i, vp, vp], i),
         "nls_rhs": ([pp, vp, vp, vp, vp, vp, vp], i),
@@ -300,6 +302,23 @@ def advance(desc, iters, pumping, psi, nres, workspace, stream=None):
 def release():
     """Free the calling thread's cached step-block graphs (nls_release)."""
     _check(load().nls_release())
+
+
+def slab_halo_rows(order):
+    """Ghost rows on each side of a row slab (nls_slab_halo_rows): 2 (order - 1)."""
+    h = load().nls_slab_halo_rows(int(order))
+    if h < 0:
+        _check(h)
+    return h
+
+
+def slab_step(desc, rows, pumping, psi_in, nres_in, psi_out, nres_out, stream=None):
+    """One RK4 step of a row slab (nls_slab_step) on torch tensors of
+    (rows + 2 h) x n points; writes the slab's own rows of psi_out/nres_out."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    _check(load().nls_slab_step(C.byref(desc), int(rows), _dp(pumping), _dp(psi_in), _dp(nres_in), _dp(psi_out),
+                                _dp(nres_out), C.c_void_p(s.cuda_stream)))
 
 
 def rhs(desc, pumping, psi, nres, dpsi, dn, stream=None):

@@ -88,3 +88,22 @@ def test_engine_field():
         native.solve_host(native.ProblemDesc.make(2, 16, 5, 1e-3, 0.1, c, engine=7), 1,
                           np.ones((16, 16)), np.ones((16, 16)))
     assert C.sizeof(native.ProblemDesc) == 8 * 4 + 2 * 8 + 23 * 8
+
+
+def test_slab_arguments_on_host():
+    # every call below fails its host-side checks before any device work
+    L = native.load()
+    assert [L.nls_slab_halo_rows(o) for o in (3, 5, 7)] == [4, 8, 12]
+    assert L.nls_slab_halo_rows(4) == native.NLS_EINVAL
+    c = np.zeros(23)
+    c[10] = 1.0
+    a, b, q = C.c_void_p(0x1000), C.c_void_p(0x2000), C.c_void_p(0x3000)
+    p = native.ProblemDesc.make(2, 64, 5, 1e-3, 0.1, c, bc="periodic")
+    step = lambda d, rows, *bufs: L.nls_slab_step(C.byref(d), rows, *bufs, None)
+    assert step(p, 7, q, a, a, b, b) == native.NLS_EINVAL                 # fewer rows than the 8 ghost rows
+    assert step(p, 32, q, a, a, a, b) == native.NLS_EINVAL                # psi in place
+    assert step(p, 32, q, a, None, b, b) == native.NLS_EINVAL             # no reservoir input
+    for bad in (native.ProblemDesc.make(2, 64, 5, 1e-3, 0.1, c, bc="ref"),
+                native.ProblemDesc.make(1, 64, 5, 1e-3, 0.1, c, bc="periodic"),
+                native.ProblemDesc.make(2, 64, 5, 1e-3, 0.1, c, bc="periodic", scheme="ssf")):
+        assert step(bad, 32, q, a, a, b, b) == native.NLS_EUNSUPPORTED
