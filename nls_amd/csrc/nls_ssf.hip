@@ -19,9 +19,9 @@
 // on chip.
 //
 // FFTs: in-place Stockham radix-4 (last stage radix-2 when log2 n is odd) on
-// up to 4096 complex values per workgroup held in LDS; each stage stages its
-// butterfly inputs in registers, synchronises, then writes the outputs in
-// place.  Twiddles exp(-2 pi i m / n) and the phase table are generated in
+// the complex values a workgroup holds in LDS (2048 in the 2-D passes, 4096
+// for a 1-D line); each stage stages its butterfly inputs and twiddles in
+// registers, synchronises, then writes the outputs in place.  Twiddles exp(-2 pi i m / n) and the phase table are generated in
 // double precision and rounded once to the working precision.
 #include <cstring>
 
@@ -30,8 +30,21 @@
 
 namespace nls {
 
-constexpr int SSF_PTS = 4096;       // complex values per workgroup
-constexpr int SSF_LOG_PTS = 12;
+// Points and threads per workgroup.  Row and column passes hold one
+// 2048-point line (or several shorter lines) per workgroup: 32 KB of LDS in
+// double precision, so up to five workgroups share a CU.  The 1-D kernel
+// holds a whole line of up to 4096 points in one workgroup (1024 threads in
+// single precision; 512 in double precision, within 256 registers).
+constexpr int ROW_PTS = 2048, ROW_LOG_PTS = 11, ROW_NTH = 256;
+constexpr int LINE_PTS = 4096;
+template <typename T> struct LineShape { static constexpr int NTH = sizeof(T) == 8 ? 512 : 1024; };
+// Column pass: 2^LOG_L adjacent columns per workgroup (fewer when n is
+// smaller), so each row segment it reads and writes is 32 bytes: two FP64
+// or four FP32 columns.  More columns per workgroup would only leave fewer
+// workgroups for small grids.
+template <typename T> struct ColShape;
+template <> struct ColShape<double> { static constexpr int LOG_L = 1, PTS = 4096, NTH = 512; };
+template <> struct ColShape<float> { static constexpr int LOG_L = 2, PTS = 8192, NTH = 1024; };
 
 template <typename T>
 __device__ __forceinline__ cplx_t<T> cmul(cplx_t<T> a, cplx_t<T> b) {
@@ -40,39 +53,46 @@ __device__ __forceinline__ cplx_t<T> cmul(cplx_t<T> a, cplx_t<T> b) {
 
 // Element m of line l lives at buf[l*lstride + m*estride].  LINE_FAST maps
 // consecutive threads to consecutive lines (column pass: lines are adjacent
-// in LDS) instead of consecutive elements (row pass).
-template <typename T, bool INV, bool LINE_FAST>
+// in LDS) instead of consecutive elements (row pass).  A stage reads its
+// butterfly inputs from LDS and its twiddles from the table (global memory,
+// cached) before the barrier, so both are in flight together; after the
+// barrier it computes and writes the outputs in place.
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
 __device__ __forceinline__ void fft_radix4_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
                                                  int logNs, const cplx_t<T> *tw) {
     using C = cplx_t<T>;
+    constexpr int QB = (PTS / 4 + NTH - 1) / NTH;    // butterflies per thread
     const int lognb = logN - 2;
-    const int total = 1 << (logL + lognb);      // butterflies in the workgroup (<= 1024)
+    const int total = 1 << (logL + lognb);           // butterflies in the workgroup
     const int Ns = 1 << logNs;
     const int sh = logN - logNs - 2;
-    C v[4][4];
+    C v[QB][4], w[QB][3];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int b = threadIdx.x + 256 * q;
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
         if (b < total) {
             const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
             const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < 4; ++r) w[q][r - 1] = tw[(r * k) << sh];
 #pragma unroll
             for (int r = 0; r < 4; ++r) v[q][r] = buf[l * lstride + (j + (r << lognb)) * estride];
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int b = threadIdx.x + 256 * q;
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
         if (b < total) {
             const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
             const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
             const int k = j & (Ns - 1);
 #pragma unroll
             for (int r = 1; r < 4; ++r) {
-                C w = tw[(r * k) << sh];
-                if (INV) w.y = -w.y;
-                v[q][r] = cmul<T>(v[q][r], w);
+                C wr = w[q][r - 1];
+                if (INV) wr.y = -wr.y;
+                v[q][r] = cmul<T>(v[q][r], wr);
             }
             const C t0 = mk<T>(v[q][0].x + v[q][2].x, v[q][0].y + v[q][2].y);
             const C t1 = mk<T>(v[q][0].x - v[q][2].x, v[q][0].y - v[q][2].y);
@@ -91,36 +111,115 @@ __device__ __forceinline__ void fft_radix4_stage(cplx_t<T> *buf, int logN, int l
     __syncthreads();
 }
 
-template <typename T, bool INV, bool LINE_FAST>
-__device__ __forceinline__ void fft_radix2_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+// Radix-8 stage: the 8-point DFT of each butterfly in registers, as two
+// 4-point DFTs (even and odd inputs) joined by the eighth roots of unity.
+template <typename T, bool INV>
+__device__ __forceinline__ void dft4(cplx_t<T> &a0, cplx_t<T> &a1, cplx_t<T> &a2, cplx_t<T> &a3) {
+    using C = cplx_t<T>;
+    const C t0 = mk<T>(a0.x + a2.x, a0.y + a2.y);
+    const C t1 = mk<T>(a0.x - a2.x, a0.y - a2.y);
+    const C t2 = mk<T>(a1.x + a3.x, a1.y + a3.y);
+    const C d13 = mk<T>(a1.x - a3.x, a1.y - a3.y);
+    const C t3 = INV ? mk<T>(-d13.y, d13.x) : mk<T>(d13.y, -d13.x);
+    a0 = mk<T>(t0.x + t2.x, t0.y + t2.y);
+    a1 = mk<T>(t1.x + t3.x, t1.y + t3.y);
+    a2 = mk<T>(t0.x - t2.x, t0.y - t2.y);
+    a3 = mk<T>(t1.x - t3.x, t1.y - t3.y);
+}
+
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
+__device__ __forceinline__ void fft_radix8_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
                                                  int logNs, const cplx_t<T> *tw) {
     using C = cplx_t<T>;
-    const int lognb = logN - 1;
-    const int total = 1 << (logL + lognb);      // <= 2048
+    constexpr int QB = (PTS / 8 + NTH - 1) / NTH;
+    const int lognb = logN - 3;
+    const int total = 1 << (logL + lognb);
     const int Ns = 1 << logNs;
-    const int sh = logN - logNs - 1;
-    C v[8][2];
+    const int sh = logN - logNs - 3;
+    C v[QB][8], w[QB][7];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int b = threadIdx.x + 256 * q;
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
         if (b < total) {
             const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
             const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < 8; ++r) w[q][r - 1] = tw[(r * k) << sh];
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[q][r] = buf[l * lstride + (j + (r << lognb)) * estride];
+        }
+    }
+    __syncthreads();
+    const T h = (T)0.70710678118654752440;     // cos(pi/4)
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < 8; ++r) {
+                C wr = w[q][r - 1];
+                if (INV) wr.y = -wr.y;
+                v[q][r] = cmul<T>(v[q][r], wr);
+            }
+            C e0 = v[q][0], e1 = v[q][2], e2 = v[q][4], e3 = v[q][6];
+            C o0 = v[q][1], o1 = v[q][3], o2 = v[q][5], o3 = v[q][7];
+            dft4<T, INV>(e0, e1, e2, e3);
+            dft4<T, INV>(o0, o1, o2, o3);
+            // o_r *= exp(-+ 2 pi i r / 8), r = 1, 2, 3
+            o1 = INV ? mk<T>((o1.x - o1.y) * h, (o1.x + o1.y) * h) : mk<T>((o1.x + o1.y) * h, (o1.y - o1.x) * h);
+            o2 = INV ? mk<T>(-o2.y, o2.x) : mk<T>(o2.y, -o2.x);
+            o3 = INV ? mk<T>(-(o3.x + o3.y) * h, (o3.x - o3.y) * h) : mk<T>((o3.y - o3.x) * h, -(o3.x + o3.y) * h);
+            const int d = ((j >> logNs) << (logNs + 3)) + k;
+            const int o = l * lstride;
+            buf[o + d * estride] = mk<T>(e0.x + o0.x, e0.y + o0.y);
+            buf[o + (d + Ns) * estride] = mk<T>(e1.x + o1.x, e1.y + o1.y);
+            buf[o + (d + 2 * Ns) * estride] = mk<T>(e2.x + o2.x, e2.y + o2.y);
+            buf[o + (d + 3 * Ns) * estride] = mk<T>(e3.x + o3.x, e3.y + o3.y);
+            buf[o + (d + 4 * Ns) * estride] = mk<T>(e0.x - o0.x, e0.y - o0.y);
+            buf[o + (d + 5 * Ns) * estride] = mk<T>(e1.x - o1.x, e1.y - o1.y);
+            buf[o + (d + 6 * Ns) * estride] = mk<T>(e2.x - o2.x, e2.y - o2.y);
+            buf[o + (d + 7 * Ns) * estride] = mk<T>(e3.x - o3.x, e3.y - o3.y);
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
+__device__ __forceinline__ void fft_radix2_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+                                                 int logNs, const cplx_t<T> *tw) {
+    using C = cplx_t<T>;
+    constexpr int QB = (PTS / 2 + NTH - 1) / NTH;
+    const int lognb = logN - 1;
+    const int total = 1 << (logL + lognb);
+    const int Ns = 1 << logNs;
+    const int sh = logN - logNs - 1;
+    C v[QB][2], w[QB];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            w[q] = tw[(j & (Ns - 1)) << sh];
             v[q][0] = buf[l * lstride + j * estride];
             v[q][1] = buf[l * lstride + (j + (1 << lognb)) * estride];
         }
     }
     __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-        const int b = threadIdx.x + 256 * q;
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
         if (b < total) {
             const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
             const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
             const int k = j & (Ns - 1);
-            C w = tw[k << sh];
-            if (INV) w.y = -w.y;
-            const C a1 = cmul<T>(v[q][1], w);
+            C wq = w[q];
+            if (INV) wq.y = -wq.y;
+            const C a1 = cmul<T>(v[q][1], wq);
             const int d = ((j >> logNs) << (logNs + 1)) + k;
             const int o = l * lstride;
             buf[o + d * estride] = mk<T>(v[q][0].x + a1.x, v[q][0].y + a1.y);
@@ -130,16 +229,17 @@ __device__ __forceinline__ void fft_radix2_stage(cplx_t<T> *buf, int logN, int l
     __syncthreads();
 }
 
-// In-place FFT of 2^logL lines of length 2^logN (2^(logL+logN) <= 4096) in LDS.
-template <typename T, bool INV, bool LINE_FAST>
+// In-place FFT of 2^logL lines of length 2^logN (2^(logL+logN) <= PTS) in LDS.
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
 __device__ __forceinline__ void lds_fft(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
                                         const cplx_t<T> *tw) {
     int logNs = 0;
-    while (logN - logNs >= 2) {
-        fft_radix4_stage<T, INV, LINE_FAST>(buf, logN, logL, lstride, estride, logNs, tw);
-        logNs += 2;
+    while (logN - logNs >= 3) {
+        fft_radix8_stage<T, INV, LINE_FAST, PTS, NTH>(buf, logN, logL, lstride, estride, logNs, tw);
+        logNs += 3;
     }
-    if (logNs < logN) fft_radix2_stage<T, INV, LINE_FAST>(buf, logN, logL, lstride, estride, logNs, tw);
+    if (logN - logNs == 2) fft_radix4_stage<T, INV, LINE_FAST, PTS, NTH>(buf, logN, logL, lstride, estride, logNs, tw);
+    if (logN - logNs == 1) fft_radix2_stage<T, INV, LINE_FAST, PTS, NTH>(buf, logN, logL, lstride, estride, logNs, tw);
 }
 
 // One RK4 step of size tau of the pointwise system at one point.
@@ -179,20 +279,25 @@ template <typename T> struct SsfArgs {
 
 enum { ROWS_FIRST = 0, ROWS_MIDDLE = 1, ROWS_LAST = 2 };
 
-// Row pass.  2^(12-logN) rows per workgroup (at most n).
+// Lines of length 2^logN per workgroup of 2^logP points (at most n lines).
+__host__ __device__ constexpr int lines_per_wg(int logN, int logP) {
+    return logP - logN < logN ? logP - logN : logN;
+}
+
+// Row pass: 2^(11-logN) rows per workgroup (at most n).
 template <typename T, bool DYN, int MODE>
-__global__ void __launch_bounds__(256) k_ssf_rows(const SsfArgs<T> a) {
+__global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<T> a) {
     using C = cplx_t<T>;
-    __shared__ C buf[SSF_PTS];
+    constexpr int PPT = ROW_PTS / ROW_NTH;
+    __shared__ C buf[ROW_PTS];
     const int logN = a.logN, N = 1 << logN;
-    int logL = SSF_LOG_PTS - logN;
-    if (logL > logN) logL = logN;
+    const int logL = lines_per_wg(logN, ROW_LOG_PTS);
     const int pts = 1 << (logL + logN);
     const size_t base = (size_t)blockIdx.x << (logL + logN);    // rows are contiguous
-    T pp[16], nn[16];
+    T pp[PPT], nn[PPT];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + ROW_NTH * q;
         pp[q] = 0; nn[q] = 0;
         if (idx < pts) {
             buf[idx] = a.psi[base + idx];
@@ -201,10 +306,10 @@ __global__ void __launch_bounds__(256) k_ssf_rows(const SsfArgs<T> a) {
         }
     }
     __syncthreads();
-    if (MODE != ROWS_FIRST) lds_fft<T, true, false>(buf, logN, logL, N, 1, a.tw);
+    if (MODE != ROWS_FIRST) lds_fft<T, true, false, ROW_PTS, ROW_NTH>(buf, logN, logL, N, 1, a.tw);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + ROW_NTH * q;
         if (idx < pts) {
             C v = buf[idx];
             T nv = nn[q];
@@ -215,10 +320,10 @@ __global__ void __launch_bounds__(256) k_ssf_rows(const SsfArgs<T> a) {
         }
     }
     __syncthreads();
-    if (MODE != ROWS_LAST) lds_fft<T, false, false>(buf, logN, logL, N, 1, a.tw);
+    if (MODE != ROWS_LAST) lds_fft<T, false, false, ROW_PTS, ROW_NTH>(buf, logN, logL, N, 1, a.tw);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + ROW_NTH * q;
         if (idx < pts) {
             a.psi[base + idx] = buf[idx];
             if (DYN) a.nres[base + idx] = nn[q];
@@ -226,43 +331,46 @@ __global__ void __launch_bounds__(256) k_ssf_rows(const SsfArgs<T> a) {
     }
 }
 
-// Column pass: 2^(12-logN) adjacent columns per workgroup, LDS [row][col].
-// Workgroups of one XCD take adjacent column blocks (round-robin dispatch).
+// Column pass: min(2^LOG_L, n) adjacent columns per workgroup, LDS
+// [row][col].  Workgroups of one XCD take adjacent column blocks
+// (round-robin dispatch).
 template <typename T>
-__global__ void __launch_bounds__(256) k_ssf_cols(const SsfArgs<T> a, int nblocks, int chunk) {
+__global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<T> a, int nblocks, int chunk) {
     using C = cplx_t<T>;
-    __shared__ C buf[SSF_PTS];
+    constexpr int COL_PTS = ColShape<T>::PTS, COL_NTH = ColShape<T>::NTH;
+    constexpr int PPT = COL_PTS / COL_NTH;
+    static_assert(COL_PTS == 2048 << ColShape<T>::LOG_L, "column block of 2048-point columns");
+    __shared__ C buf[COL_PTS];
     const int b = blockIdx.x;
     const int blk = (b & 7) * chunk + (b >> 3);
     if (blk >= nblocks) return;
     const int logN = a.logN, N = 1 << logN;
-    int logL = SSF_LOG_PTS - logN;
-    if (logL > logN) logL = logN;
+    const int logL = ColShape<T>::LOG_L < logN ? ColShape<T>::LOG_L : logN;
     const int L = 1 << logL, pts = L << logN;
     const int col0 = blk << logL;
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + COL_NTH * q;
         if (idx < pts) {
             const int m = idx >> logL, cc = idx & (L - 1);
             buf[idx] = a.psi[(size_t)m * N + col0 + cc];
         }
     }
     __syncthreads();
-    lds_fft<T, false, true>(buf, logN, logL, 1, L, a.tw);
+    lds_fft<T, false, true, COL_PTS, COL_NTH>(buf, logN, logL, 1, L, a.tw);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + COL_NTH * q;
         if (idx < pts) {
             const int m = idx >> logL, cc = idx & (L - 1);
             buf[idx] = cmul<T>(buf[idx], cmul<T>(a.ph[m], a.ph[col0 + cc]));
         }
     }
     __syncthreads();
-    lds_fft<T, true, true>(buf, logN, logL, 1, L, a.tw);
+    lds_fft<T, true, true, COL_PTS, COL_NTH>(buf, logN, logL, 1, L, a.tw);
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + COL_NTH * q;
         if (idx < pts) {
             const int m = idx >> logL, cc = idx & (L - 1);
             a.psi[(size_t)m * N + col0 + cc] = buf[idx];
@@ -273,14 +381,16 @@ __global__ void __launch_bounds__(256) k_ssf_cols(const SsfArgs<T> a, int nblock
 // 1-D: one workgroup holds the whole line in LDS and runs every step; the
 // pumping and reservoir of the thread's points stay in registers.
 template <typename T, bool DYN>
-__global__ void __launch_bounds__(256) k_ssf_line(const SsfArgs<T> a) {
+__global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<T> a) {
     using C = cplx_t<T>;
-    __shared__ C buf[SSF_PTS];
+    constexpr int LINE_NTH = LineShape<T>::NTH;
+    constexpr int PPT = LINE_PTS / LINE_NTH;
+    __shared__ C buf[LINE_PTS];
     const int logN = a.logN, N = 1 << logN;
-    T pp[16], nn[16];
+    T pp[PPT], nn[PPT];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + LINE_NTH * q;
         pp[q] = 0; nn[q] = 0;
         if (idx < N) {
             buf[idx] = a.psi[idx];
@@ -292,8 +402,8 @@ __global__ void __launch_bounds__(256) k_ssf_line(const SsfArgs<T> a) {
         // each thread only touches its own points here, so no barrier is
         // needed before the first FFT stage (which starts with one)
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int idx = threadIdx.x + 256 * q;
+        for (int q = 0; q < PPT; ++q) {
+            const int idx = threadIdx.x + LINE_NTH * q;
             if (idx < N) {
                 C v = buf[idx];
                 pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[q], a.half_dt, v, nn[q]);
@@ -301,17 +411,17 @@ __global__ void __launch_bounds__(256) k_ssf_line(const SsfArgs<T> a) {
             }
         }
         __syncthreads();
-        lds_fft<T, false, false>(buf, logN, 0, N, 1, a.tw);
+        lds_fft<T, false, false, LINE_PTS, LINE_NTH>(buf, logN, 0, N, 1, a.tw);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int idx = threadIdx.x + 256 * q;
+        for (int q = 0; q < PPT; ++q) {
+            const int idx = threadIdx.x + LINE_NTH * q;
             if (idx < N) buf[idx] = cmul<T>(buf[idx], a.ph[idx]);
         }
         __syncthreads();
-        lds_fft<T, true, false>(buf, logN, 0, N, 1, a.tw);
+        lds_fft<T, true, false, LINE_PTS, LINE_NTH>(buf, logN, 0, N, 1, a.tw);
 #pragma unroll
-        for (int q = 0; q < 16; ++q) {
-            const int idx = threadIdx.x + 256 * q;
+        for (int q = 0; q < PPT; ++q) {
+            const int idx = threadIdx.x + LINE_NTH * q;
             if (idx < N) {
                 C v = buf[idx];
                 pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[q], a.half_dt, v, nn[q]);
@@ -321,8 +431,8 @@ __global__ void __launch_bounds__(256) k_ssf_line(const SsfArgs<T> a) {
         __syncthreads();
     }
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-        const int idx = threadIdx.x + 256 * q;
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + LINE_NTH * q;
         if (idx < N) {
             a.psi[idx] = buf[idx];
             if (DYN) a.nres[idx] = nn[q];
@@ -394,18 +504,19 @@ static int ssf_run(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, 
     a.psi = psi; a.nres = nres; a.P = P; a.tw = tw; a.ph = ph;
     if (p->dim == 1) {
         a.iters = iters;
-        hipLaunchKernelGGL((k_ssf_line<T, DYN>), dim3(1), dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_ssf_line<T, DYN>), dim3(1), dim3(LineShape<T>::NTH), 0, s, a);
         return hip_check(hipGetLastError(), "ssf line launch");
     }
-    int logL = SSF_LOG_PTS - logN;
-    if (logL > logN) logL = logN;
-    const int blocks = N >> logL;
-    const int chunk = (blocks + 7) / 8;
-    hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_FIRST>), dim3(blocks), dim3(256), 0, s, a);
+    const int rblocks = N >> lines_per_wg(logN, ROW_LOG_PTS);
+    const int cblocks = N >> (ColShape<T>::LOG_L < logN ? ColShape<T>::LOG_L : logN);
+    const int chunk = (cblocks + 7) / 8;
+    hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_FIRST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(k_ssf_cols<T>, dim3(8 * chunk), dim3(256), 0, s, a, blocks, chunk);
-        if (it + 1 < iters) hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_MIDDLE>), dim3(blocks), dim3(256), 0, s, a);
-        else hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_LAST>), dim3(blocks), dim3(256), 0, s, a);
+        hipLaunchKernelGGL(k_ssf_cols<T>, dim3(8 * chunk), dim3(ColShape<T>::NTH), 0, s, a, cblocks, chunk);
+        if (it + 1 < iters)
+            hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_MIDDLE>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
+        else
+            hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_LAST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
     }
     return hip_check(hipGetLastError(), "ssf launch");
 }
