@@ -20,6 +20,8 @@ schedulr/workr.py:39-107).  Here requests share the GPU:
   finishes mid-block holds its state and its slot is freed at the end of the
   block, when waiting requests are admitted; the block length adapts so one
   block takes about target_block_ms;
+* each pool runs on its own stream (default priority), so the 1-D lines,
+  small grids and large grids of a block run on the device at the same time;
 * a member's arithmetic never depends on the other members, so a result is
   bit-identical alone and in any mix.
 
@@ -112,6 +114,9 @@ class Group(object):
         self.members = []
         self.handle = None
         self.dirty = True
+        # the pool's own stream (default priority): the pools' launches
+        # overlap on the device; admission fills and steps are ordered on it
+        self.stream = torch.cuda.Stream(device=device)
 
     @staticmethod
     def bytes_per_point(engine, dtype):
@@ -188,7 +193,7 @@ class Group(object):
         done = [r for r in self.members if r.remaining == 0]
         if not done:
             return []
-        self.torch.cuda.current_stream().synchronize()
+        self.stream.synchronize()
         for r in done:
             a, b = r.offset, r.offset + r.points
             shape = (r.n,) if r.dim == 1 else (r.n, r.n)
@@ -240,14 +245,16 @@ class Engine(object):
         pending = list(requests)
         waiting = []
         finished = []
-        stream = self.torch.cuda.current_stream()
         while pending or waiting or any(g.members for g in self.groups.values()):
             t = now()
             while pending and pending[0].arrival <= t:
                 waiting.append(pending.pop(0))
             still = []
             for r in waiting:                       # arrival order; later ones may pass
-                if self.groups[self.group_of(r)].admit(r):
+                g = self.groups[self.group_of(r)]
+                with self.torch.cuda.stream(g.stream):
+                    ok = g.admit(r)
+                if ok:
                     r.admit_time = now()
                 else:
                     still.append(r)
@@ -258,11 +265,12 @@ class Engine(object):
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
                 continue
             tb = time.perf_counter()
+            for g in active:                        # each pool on its own stream, concurrently
+                g.advance(self.block, g.stream)
             for g in active:
-                g.advance(self.block, stream)
+                g.stream.synchronize()
             for g in active:
                 finished += g.retire(now)
-            stream.synchronize()
             self.blocks += 1
             el = (time.perf_counter() - tb) * 1e3
             per_step = el / max(self.block, 1)
