@@ -109,7 +109,11 @@ def ssf_case(dim, n, dyn, seed=4):
     return h, P, psi0, n0, c
 
 
-CASES = [(1, 64), (1, 512), (1, 4096), (2, 16), (2, 128), (2, 512)]
+# every FFT length class: radix-8 stages with a radix-2 or radix-4 tail or
+# none (double precision), radix-4 with or without a radix-2 tail (single),
+# several rows / columns per workgroup (n < 2048) and one (n = 2048)
+CASES = [(1, 16), (1, 64), (1, 512), (1, 2048), (1, 4096),
+         (2, 16), (2, 32), (2, 64), (2, 128), (2, 256), (2, 512), (2, 1024)]
 
 
 @pytest.mark.gpu
@@ -129,7 +133,7 @@ def test_gpu_ssf_f64_matches_oracle(gpu, dim, n, dyn):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dim,n", [(1, 1024), (2, 256), (2, 2048)])
+@pytest.mark.parametrize("dim,n", [(1, 32), (1, 1024), (1, 4096), (2, 16), (2, 64), (2, 256), (2, 2048)])
 def test_gpu_ssf_f32_within_tolerance(gpu, dim, n):
     from nls_amd import native
     h, P, psi0, n0, c = ssf_case(dim, n, True)
