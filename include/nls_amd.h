@@ -76,7 +76,8 @@ int nls_version(int *major, int *minor, int *patch);
 
 /* solve_nls (nls/nls.f90:797-813) and solve_nls_1d (:815-827): 1-D
  * axisymmetric solve, order 3/5/7, iters RK4 steps of dt on r_k = k*dx.
- * pumping[n] float, coeffs[23] float, u0[n] and u[n] complex64. */
+ * pumping[n] float, coeffs[23] float, u0[n] and u[n] complex64.  Like the
+ * reference, the reservoir is algebraic: coeffs[10] (c11) is not read. */
 int nls_solve_nls(float dt, float dx, int n, int order, int iters, const float *pumping,
                   const float *coeffs, const void *u0, void *u);
 int nls_solve_nls_1d(float dt, float dx, int n, int order, int iters, const float *pumping,
@@ -97,6 +98,45 @@ int nls_chemical_potential_2d(float dx, int n, const float *pumping, const float
  * storage, Fortran shape (order, n) -> op[(j-1)*order + (p-1)].  Built in
  * double precision on the host. */
 int nls_make_laplacian(int n, int order, double h, double *op);
+
+/* The reference module's other public procedures (nls.f90:13-24).  Fortran
+ * arrays are column-major: a(p, j) at a[(j-1)*rows + (p-1)].
+ *
+ * Operator builders and BLAS-2 wrappers, host code as in the reference:
+ *   make_banded_matrix (:61-91): mat(m, n) from row(m);
+ *   make_laplacian_2d (:297-403): blocks(n, 2m-1) and orders(m) of the 2-D
+ *     Cartesian operator (the reference declares order 7 blocks(m, 2m-1),
+ *     valid only at n = 7; here (n, 2m-1) at every n);
+ *   rgbmv (:530-541): u += sign A x, A = op(2 klu + 1, n) in band storage;
+ *   rbbmv (:408-527): y += sign B x for the block-banded B (blocks, ms);
+ *   revervoir / revervoir_2d (:570-581, :829-839):
+ *     r = c12 P / (c13 + c14 u_sqr) on n or n*n points. */
+int nls_make_banded_matrix(int n, int m, const float *row, float *mat);
+int nls_make_laplacian_2d(int n, int m, float h, float *blocks, int *orders);
+int nls_rgbmv(const float *x, float *u, float sign, const float *op, int klu, int n);
+int nls_rbbmv(const float *x, float *y, float sign, const float *blocks, const int *ms, int m, int n);
+int nls_revervoir(const float *pumping, const float *coeffs, const float *u_sqr, float *r, int n);
+int nls_revervoir_2d(const float *pumping, const float *coeffs, const float *u_sqr, float *r, int n);
+
+/* Right-hand side and RK4 on a caller-given operator, run on the GPU (the
+ * per-stage kernels), host arrays, algebraic reservoir:
+ *   hamiltonian (:621-650): v = F(u) with the band op(2 klu + 1, n),
+ *     klu = 1, 2 or 3 (any band of that width);
+ *   runge_kutta (:705-734): iters RK4 steps of dt from u0 with op(order, n);
+ *     t0 is accepted and, as in the reference, unused;
+ *   hamiltonian_2d (:841-870), runge_kutta_2d (:873-901): the same on n x n
+ *     with (blocks, orders); the blocks must hold one symmetric
+ *     constant-coefficient Cartesian stencil with zero outside the grid, as
+ *     make_laplacian_2d builds (the kernels apply such stencils); other
+ *     block operators give NLS_EUNSUPPORTED. */
+int nls_hamiltonian(const float *pumping, const float *coeffs, const void *u, void *v, const float *op,
+                    int klu, int n);
+int nls_runge_kutta(float dt, float t0, const void *u0, const float *op, int n, int order, int iters,
+                    void *u, const float *pumping, const float *coeffs);
+int nls_hamiltonian_2d(const float *pumping, const float *coeffs, const void *u, void *v,
+                       const float *blocks, const int *orders, int order, int n);
+int nls_runge_kutta_2d(float dt, float t0, const void *u0, int n, const float *blocks, const int *orders,
+                       int order, int iters, void *u, const float *pumping, const float *coeffs);
 
 /* ======================================================================
  * Extended entry points: any precision, boundary family, reservoir mode

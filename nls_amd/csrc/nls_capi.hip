@@ -89,9 +89,39 @@ static void fill_weights(const nls_problem *p, StageArgs<T> &a) {
     a.w0 = w[0]; a.w1 = w[1]; a.w2 = w[2]; a.w3 = w[3];
 }
 
+// A caller-given operator in place of the one built from (order, dx): the
+// row table of a 1-D band (n x order, row i holding the weights of columns
+// i - R .. i + R) or the weights w0 .. wR of a 2-D Cartesian stencil.
+struct OperatorGiven {
+    const double *tab = nullptr;
+    const double *w = nullptr;
+};
+
+// Operator of the stage kernels: weights (and, for the 1-D radial operator,
+// its row table in the workspace slot `tab`).  Returns an error code.
+template <typename T>
+static int set_operator(const nls_problem *p, StageArgs<T> &a, T *tab, const OperatorGiven *og, hipStream_t s) {
+    fill_weights<T>(p, a);
+    if (og && og->w) {
+        a.w0 = (T)og->w[0]; a.w1 = (T)og->w[1]; a.w2 = (T)og->w[2]; a.w3 = (T)og->w[3];
+    }
+    if (p->dim == 1 && p->bc == NLS_BC_REF) {
+        if (og && og->tab) {
+            std::vector<T> t((size_t)p->n * p->order);
+            for (size_t i = 0; i < t.size(); ++i) t[i] = (T)og->tab[i];
+            int rc = hip_check(hipMemcpy(tab, t.data(), t.size() * sizeof(T), hipMemcpyHostToDevice), "upload operator");
+            if (rc) return rc;
+        } else {
+            launch_radial_table<T>(p->n, p->order, p->dx, tab, s);
+        }
+        a.wtab = tab;
+    }
+    return NLS_OK;
+}
+
 template <typename T>
 static int rk4_advance(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws,
-                       hipStream_t s) {
+                       hipStream_t s, const OperatorGiven *og = nullptr) {
     using C = cplx_t<T>;
     const size_t N = points_of(p);
     const bool dyn = p->coeffs[10] != 0.0;
@@ -111,12 +141,8 @@ static int rk4_advance(const nls_problem *p, int iters, const T *P, cplx_t<T> *p
     a.dt = (T)p->dt;
     a.c = make_coeffs<T>(p->coeffs);
     a.P = P;
-    fill_weights<T>(p, a);
-    if (p->dim == 1 && p->bc == NLS_BC_REF) {
-        T *tab = (T *)(ws + off_tab);
-        launch_radial_table<T>(p->n, p->order, p->dx, tab, s);
-        a.wtab = tab;
-    }
+    int rc = set_operator<T>(p, a, (T *)(ws + off_tab), og, s);
+    if (rc) return rc;
     C *cur = psi, *alt = acc;
     T *curn = nres, *altn = accn;
     for (int it = 0; it < iters; ++it) {
@@ -132,7 +158,7 @@ static int rk4_advance(const nls_problem *p, int iters, const T *P, cplx_t<T> *p
         std::swap(cur, alt);
         std::swap(curn, altn);
     }
-    int rc = hip_check(hipGetLastError(), "RK4 stage launch");
+    rc = hip_check(hipGetLastError(), "RK4 stage launch");
     if (rc) return rc;
     if (cur != psi) {
         rc = hip_check(hipMemcpyAsync(psi, cur, N * sizeof(C), hipMemcpyDeviceToDevice, s), "copy state");
@@ -144,7 +170,7 @@ static int rk4_advance(const nls_problem *p, int iters, const T *P, cplx_t<T> *p
 
 template <typename T>
 static int rhs_once(const nls_problem *p, const T *P, const cplx_t<T> *psi, const T *nres, cplx_t<T> *dpsi,
-                    T *dn, char *ws, hipStream_t s) {
+                    T *dn, char *ws, hipStream_t s, const OperatorGiven *og = nullptr) {
     const bool dyn = p->coeffs[10] != 0.0;
     StageArgs<T> a;
     memset(&a, 0, sizeof a);
@@ -152,11 +178,8 @@ static int rhs_once(const nls_problem *p, const T *P, const cplx_t<T> *psi, cons
     a.dt = (T)p->dt;
     a.c = make_coeffs<T>(p->coeffs);
     a.P = P;
-    fill_weights<T>(p, a);
-    if (p->dim == 1 && p->bc == NLS_BC_REF) {
-        launch_radial_table<T>(p->n, p->order, p->dx, (T *)ws, s);
-        a.wtab = (T *)ws;
-    }
+    int rc = set_operator<T>(p, a, (T *)ws, og, s);
+    if (rc) return rc;
     a.stage = 1; a.u = psi; a.un = nres; a.kout = dpsi; a.kout_n = dn;
     launch_rk4_stage<T>(p->dim, p->order, p->bc, dyn, a, s);
     return hip_check(hipGetLastError(), "RHS launch");
@@ -421,13 +444,12 @@ static int dropin_solve(int dim, float dt, float dx, int n, int order, int iters
     p.engine = NLS_ENGINE_AUTO;
     p.dt = dt; p.dx = dx;
     for (int k = 0; k < 23; ++k) p.coeffs[k] = coeffs[k];
+    p.coeffs[10] = 0.0;       // the reference never reads c11: its reservoir is algebraic (nls.f90:570-581)
     int rc = check_problem(&p);
     if (rc) return rc;
     const size_t N = points_of(&p);
-    std::vector<float> nres;
-    if (p.coeffs[10] != 0.0) nres.assign(N, 0.0f);   // dynamic reservoir starts empty
     if (u != u0) memcpy(u, u0, N * 2 * sizeof(float));
-    return nls_solve_host(&p, iters, pumping, u, nres.empty() ? nullptr : nres.data());
+    return nls_solve_host(&p, iters, pumping, u, nullptr);
 }
 
 int nls_solve_nls(float dt, float dx, int n, int order, int iters, const float *pumping, const float *coeffs,
@@ -443,6 +465,137 @@ int nls_solve_nls_1d(float dt, float dx, int n, int order, int iters, const floa
 int nls_solve_nls_2d(float dt, float dx, int n, int order, int iters, const float *pumping, const float *coeffs,
                      const void *u0, void *u) {
     return dropin_solve(2, dt, dx, n, order, iters, pumping, coeffs, u0, u);
+}
+
+// ---- the reference's runge_kutta / hamiltonian family -------------------
+// Same right-hand side and RK4 as solve_nls, on an operator the caller gives
+// as the reference's arrays: a BLAS band op(2 klu + 1, n) in 1-D, or the
+// block-banded (blocks(n, 2m - 1), orders(m)) in 2-D.  Single precision,
+// host memory, blocking; the arithmetic runs on the per-stage kernels.
+
+// Row table of the 1-D kernels from a band: row i holds A(i, i - klu + k),
+// stored at op((klu + i - j) + 1, j + 1) (BLAS band layout, nls.f90:540).
+static void band_rows(int n, int klu, const float *op, std::vector<double> &tab) {
+    const int m = 2 * klu + 1;
+    tab.assign((size_t)n * m, 0.0);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < m; ++k) {
+            const int j = i + k - klu;
+            if (j >= 0 && j < n) tab[(size_t)i * m + k] = op[(size_t)j * m + (m - 1 - k)];
+        }
+}
+
+// Stencil weights w0 .. wR of a 2-D block-banded operator, provided it is
+// one symmetric constant-coefficient Cartesian stencil with zero outside the
+// grid (what make_laplacian_2d builds, nls.f90:297-403): the kernels apply
+// such a stencil; other block operators are rejected.
+static int block_stencil(int n, int m, const float *blocks, const int *orders, double w[4]) {
+    const int R = (m - 1) / 2;
+    if (n < m) return fail(NLS_EINVAL, "n=%d is smaller than the stencil width %d", n, m);
+    for (int q = 0; q < m; ++q)
+        if (orders[q] != (q == R ? R : 0))
+            return fail(NLS_EUNSUPPORTED, "orders must be those of make_laplacian_2d (block %d has %d)", q, orders[q]);
+    const float *mid = blocks + (size_t)R * n;       // band (m, n) of the diagonal block
+    std::vector<float> c(m);
+    for (int b = 0; b < m; ++b) c[b] = mid[(size_t)R * m + b];
+    const char *msg = "the block operator is not one symmetric constant-coefficient stencil";
+    for (int j = 0; j < n; ++j)
+        for (int b = 0; b < m; ++b) {
+            const int i = j + b - R;
+            if (i >= 0 && i < n && mid[(size_t)j * m + b] != c[b]) return fail(NLS_EUNSUPPORTED, "%s", msg);
+        }
+    w[0] = w[1] = w[2] = w[3] = 0.0;
+    for (int d = 1; d <= R; ++d) {
+        const float *left = blocks + (size_t)(R - d) * n, *right = blocks + (size_t)(R + m - 1 + d) * n;
+        if (c[R - d] != c[R + d]) return fail(NLS_EUNSUPPORTED, "%s", msg);
+        for (int j = 0; j < n; ++j)
+            if (left[j] != c[R - d] || right[j] != c[R - d]) return fail(NLS_EUNSUPPORTED, "%s", msg);
+        w[d] = c[R - d];
+    }
+    w[0] = 0.5 * (double)c[R];      // the kernels take the 1-D centre weight; the 2-D centre is twice it
+    return NLS_OK;
+}
+
+static int operator_dropin(int dim, int n, int order, float dt, int iters, const float *pumping,
+                           const float *coeffs, const OperatorGiven &og, const void *u0, void *u, bool rhs_only) {
+    if (!pumping || !coeffs || !u0 || !u) return fail(NLS_EINVAL, "NULL argument");
+    if (iters < 0) return fail(NLS_EINVAL, "iters must be >= 0");
+    nls_problem p;
+    memset(&p, 0, sizeof p);
+    p.dim = dim; p.n = n; p.order = order; p.bc = NLS_BC_REF; p.dtype = NLS_F32; p.scheme = NLS_SCHEME_RK4;
+    p.engine = NLS_ENGINE_STAGE;
+    p.dt = dt; p.dx = 1.0;              // the operator is given; dx is not used
+    for (int k = 0; k < 23; ++k) p.coeffs[k] = coeffs[k];
+    p.coeffs[10] = 0.0;                 // algebraic reservoir, as the reference's hamiltonian
+    int rc = check_problem(&p);
+    if (rc) return rc;
+    const size_t N = points_of(&p);
+    DevBuf dP, du, dv, ws;
+    if ((rc = dP.alloc(N * 4, "pumping")) || (rc = du.alloc(N * 8, "psi")) ||
+        (rc = ws.alloc(rk4_workspace(&p, nullptr), "workspace")) || (rhs_only && (rc = dv.alloc(N * 8, "result"))))
+        return rc;
+    if ((rc = hip_check(hipMemcpy(dP.ptr, pumping, N * 4, hipMemcpyHostToDevice), "upload pumping")) ||
+        (rc = hip_check(hipMemcpy(du.ptr, u0, N * 8, hipMemcpyHostToDevice), "upload psi")))
+        return rc;
+    if (rhs_only)
+        rc = rhs_once<float>(&p, (const float *)dP.ptr, (const float2 *)du.ptr, nullptr, (float2 *)dv.ptr, nullptr,
+                             (char *)ws.ptr, nullptr, &og);
+    else
+        rc = rk4_advance<float>(&p, iters, (const float *)dP.ptr, (float2 *)du.ptr, nullptr, (char *)ws.ptr, nullptr,
+                                &og);
+    if (rc) return rc;
+    if ((rc = hip_check(hipDeviceSynchronize(), "solve"))) return rc;
+    return hip_check(hipMemcpy(u, rhs_only ? dv.ptr : du.ptr, N * 8, hipMemcpyDeviceToHost), "download");
+}
+
+int nls_hamiltonian(const float *pumping, const float *coeffs, const void *u, void *v, const float *op, int klu,
+                    int n) {
+    if (!op) return fail(NLS_EINVAL, "NULL argument");
+    if (klu < 1 || klu > 3) return fail(NLS_EUNSUPPORTED, "klu must be 1, 2 or 3 (got %d)", klu);
+    if (n < 2 * klu + 1) return fail(NLS_EINVAL, "n=%d is smaller than the band width %d", n, 2 * klu + 1);
+    std::vector<double> tab;
+    band_rows(n, klu, op, tab);
+    OperatorGiven og;
+    og.tab = tab.data();
+    return operator_dropin(1, n, 2 * klu + 1, 0.0f, 0, pumping, coeffs, og, u, v, true);
+}
+
+int nls_runge_kutta(float dt, float t0, const void *u0, const float *op, int n, int order, int iters, void *u,
+                    const float *pumping, const float *coeffs) {
+    (void)t0;                           // advanced but never read by the reference (nls.f90:719-732)
+    if (!op) return fail(NLS_EINVAL, "NULL argument");
+    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7 (got %d)", order);
+    if (n < order) return fail(NLS_EINVAL, "n=%d is smaller than the band width %d", n, order);
+    std::vector<double> tab;
+    band_rows(n, (order - 1) / 2, op, tab);
+    OperatorGiven og;
+    og.tab = tab.data();
+    return operator_dropin(1, n, order, dt, iters, pumping, coeffs, og, u0, u, false);
+}
+
+int nls_hamiltonian_2d(const float *pumping, const float *coeffs, const void *u, void *v, const float *blocks,
+                       const int *orders, int order, int n) {
+    if (!blocks || !orders) return fail(NLS_EINVAL, "NULL argument");
+    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7 (got %d)", order);
+    double w[4];
+    int rc = block_stencil(n, order, blocks, orders, w);
+    if (rc) return rc;
+    OperatorGiven og;
+    og.w = w;
+    return operator_dropin(2, n, order, 0.0f, 0, pumping, coeffs, og, u, v, true);
+}
+
+int nls_runge_kutta_2d(float dt, float t0, const void *u0, int n, const float *blocks, const int *orders, int order,
+                       int iters, void *u, const float *pumping, const float *coeffs) {
+    (void)t0;
+    if (!blocks || !orders) return fail(NLS_EINVAL, "NULL argument");
+    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7 (got %d)", order);
+    double w[4];
+    int rc = block_stencil(n, order, blocks, orders, w);
+    if (rc) return rc;
+    OperatorGiven og;
+    og.w = w;
+    return operator_dropin(2, n, order, dt, iters, pumping, coeffs, og, u0, u, false);
 }
 
 int nls_tune_stage_tile(int dtype, int rows) {

@@ -34,6 +34,8 @@ EXPORTS = (
     "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
     "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance", "nls_tune_stage_tile",
     "nls_release", "nls_fill_profile", "nls_fill_state", "nls_slab_halo_rows", "nls_slab_step",
+    "nls_make_banded_matrix", "nls_make_laplacian_2d", "nls_rgbmv", "nls_rbbmv", "nls_revervoir",
+    "nls_revervoir_2d", "nls_hamiltonian", "nls_hamiltonian_2d", "nls_runge_kutta", "nls_runge_kutta_2d",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2}
 FLAG_EAGER = 1
@@ -136,6 +138,16 @@ def load():
         "nls_advance": ([pp, i, vp, vp, vp, vp, sz, vp], i),
         "nls_release": ([], i),
         "nls_slab_halo_rows": ([i], i),
+        "nls_make_banded_matrix": ([i, i, vp, vp], i),
+        "nls_make_laplacian_2d": ([i, i, f, vp, vp], i),
+        "nls_rgbmv": ([vp, vp, f, vp, i, i], i),
+        "nls_rbbmv": ([vp, vp, f, vp, vp, i, i], i),
+        "nls_revervoir": ([vp, vp, vp, vp, i], i),
+        "nls_revervoir_2d": ([vp, vp, vp, vp, i], i),
+        "nls_hamiltonian": ([vp, vp, vp, vp, vp, i, i], i),
+        "nls_hamiltonian_2d": ([vp, vp, vp, vp, vp, vp, i, i], i),
+        "nls_runge_kutta": ([f, f, vp, vp, i, i, i, vp, vp, vp], i),
+        "nls_runge_kutta_2d": ([f, f, vp, i, vp, vp, i, i, vp, vp, vp], i),
         "nls_slab_step": ([pp, i, vp, vp, vp, vp, vp, vp], i),
         "nls_fill_profile": ([C.POINTER(ProfileDesc), i, i, C.POINTER(GridDesc), i, vp, vp], i),
         "nls_fill_state": ([C.POINTER(ProfileDesc), d, C.c_uint64, i, i, C.POINTER(GridDesc), i, vp, vp], i),
@@ -174,10 +186,12 @@ def _ptr(a):
     return a.ctypes.data_as(C.c_void_p)
 
 
-def _arr(x, dtype, name, pos, shape=None):
-    """f2py-style conversion: cast, make contiguous, check the fixed shape."""
+def _arr(x, dtype, name, pos, shape=None, order="C"):
+    """f2py-style conversion: cast, make contiguous (order "F" for the
+    reference's 2-D operator arrays, whose memory the C ABI reads
+    column-major), check the fixed shape."""
     try:
-        a = np.array(x, dtype=dtype, copy=True, order="C")
+        a = np.array(x, dtype=dtype, copy=True, order=order)
     except (TypeError, ValueError) as e:
         raise ValueError("failed in converting %s argument `%s' to C/Fortran array: %s" % (pos, name, e))
     if shape is not None:
@@ -250,6 +264,167 @@ class _NlsModule(object):
         op = np.zeros((int(n), int(m)), np.float64)   # C (n, m) == Fortran (m, n)
         _check(load().nls_make_laplacian(int(n), int(m), float(h), _ptr(op)))
         return op.T.astype(np.float32)
+
+    # The module's other public procedures (nls.f90:13-24).  Argument lists
+    # follow f2py's rules: intent(out) arrays are returned, intent(inout)
+    # arrays (rgbmv's u, rbbmv's y) are updated in place and must be float32
+    # contiguous arrays, dimension arguments that an input array's shape
+    # fixes are optional and come last.  clear_first_row_of_derivative and
+    # divide_derivative_on_radius read their intent(out) array before writing
+    # it (nls.f90:93-130): there is no meaningful binding, and none is made.
+
+    def make_laplacian_o3(self, n, h):
+        return self.make_laplacian(n, 3, h)
+
+    def make_laplacian_o5(self, n, h):
+        return self.make_laplacian(n, 5, h)
+
+    def make_laplacian_o7(self, n, h):
+        return self.make_laplacian(n, 7, h)
+
+    @staticmethod
+    def make_banded_matrix(n, row, m=None):
+        """mat = make_banded_matrix(n, row[, m]) -- nls.f90:61, mat(m, n)."""
+        r = _arr(row, np.float32, "row", "2nd")
+        m = r.shape[0] if m is None else int(m)
+        r = _arr(r, np.float32, "row", "2nd", (m,))
+        mat = np.zeros((m, int(n)), np.float32, order="F")
+        _check(load().nls_make_banded_matrix(int(n), m, _ptr(r), _ptr(mat)))
+        return mat
+
+    @staticmethod
+    def make_laplacian_2d(n, m, h):
+        """blocks, orders = make_laplacian_2d(n, m, h) -- nls.f90:388, blocks(n, 2m-1)."""
+        n, m = int(n), int(m)
+        blocks = np.zeros((n, 2 * m - 1), np.float32, order="F")
+        orders = np.zeros(m, np.int32)
+        _check(load().nls_make_laplacian_2d(n, m, float(h), _ptr(blocks), _ptr(orders)))
+        return blocks, orders
+
+    def make_laplacian_2d_o3(self, n, h):
+        return self.make_laplacian_2d(n, 3, h)
+
+    def make_laplacian_2d_o5(self, n, h):
+        return self.make_laplacian_2d(n, 5, h)
+
+    def make_laplacian_2d_o7(self, n, h):
+        return self.make_laplacian_2d(n, 7, h)
+
+    @staticmethod
+    def _inout(y, name, pos, size):
+        if not (isinstance(y, np.ndarray) and y.dtype == np.float32 and y.size == size and
+                (y.flags.c_contiguous or y.flags.f_contiguous)):
+            raise ValueError("%s argument `%s' must be a contiguous float32 array of %d elements (intent(inout))"
+                             % (pos, name, size))
+        return y
+
+    def rgbmv(self, x, u, sign, op, klu, n=None):
+        """rgbmv(x, u, sign, op, klu[, n]) -- nls.f90:530: u += sign A x in place, A = op(2 klu + 1, n)."""
+        xa = _arr(x, np.float32, "x", "1st")
+        n = xa.shape[0] if n is None else int(n)
+        xa = _arr(xa, np.float32, "x", "1st", (n,))
+        ua = self._inout(u, "u", "2nd", n)
+        oa = _arr(op, np.float32, "op", "4th", (2 * int(klu) + 1, n), order="F")
+        _check(load().nls_rgbmv(_ptr(xa), _ptr(ua), float(sign), _ptr(oa), int(klu), n))
+
+    def rbbmv(self, x, y, sign, blocks, ms, m=None, n=None):
+        """rbbmv(x, y, sign, blocks, ms[, m, n]) -- nls.f90:510: y += sign B x in place on n*n points."""
+        msa = _arr(ms, np.int32, "ms", "5th")
+        m = msa.shape[0] if m is None else int(m)
+        ba = _arr(blocks, np.float32, "blocks", "4th", order="F")
+        n = ba.shape[0] if n is None else int(n)
+        ba = _arr(ba, np.float32, "blocks", "4th", (n, 2 * m - 1), order="F")
+        msa = _arr(msa, np.int32, "ms", "5th", (m,))
+        xa = _arr(np.ravel(x), np.float32, "x", "1st", (n * n,))
+        ya = self._inout(y, "y", "2nd", n * n)
+        _check(load().nls_rbbmv(_ptr(xa), _ptr(ya), float(sign), _ptr(ba), _ptr(msa), m, n))
+
+    def rbbmv_o3(self, x, y, sign, blocks, ms, n=None):
+        return self.rbbmv(x, y, sign, blocks, ms, 3, n)
+
+    def rbbmv_o5(self, x, y, sign, blocks, ms, n=None):
+        return self.rbbmv(x, y, sign, blocks, ms, 5, n)
+
+    def rbbmv_o7(self, x, y, sign, blocks, ms, n=None):
+        return self.rbbmv(x, y, sign, blocks, ms, 7, n)
+
+    @staticmethod
+    def _reservoir(fn, dim, pumping, coeffs, u_sqr, n):
+        P = _arr(pumping, np.float32, "pumping", "1st")
+        n = P.shape[0] if n is None else int(n)
+        shape = (n,) if dim == 1 else (n, n)
+        P = _arr(P, np.float32, "pumping", "1st", shape)
+        co = _arr(coeffs, np.float32, "coeffs", "2nd", (23,))
+        us = _arr(u_sqr, np.float32, "u_sqr", "3rd", shape)
+        r = np.empty(shape, np.float32)
+        _check(fn(_ptr(P), _ptr(co), _ptr(us), _ptr(r), n))
+        return r
+
+    def revervoir(self, pumping, coeffs, u_sqr, n=None):
+        """r = revervoir(pumping, coeffs, u_sqr[, n]) -- nls.f90:570."""
+        return self._reservoir(load().nls_revervoir, 1, pumping, coeffs, u_sqr, n)
+
+    def revervoir_2d(self, pumping, coeffs, u_sqr, n=None):
+        """r = revervoir_2d(pumping, coeffs, u_sqr[, n]) -- nls.f90:829."""
+        return self._reservoir(load().nls_revervoir_2d, 2, pumping, coeffs, u_sqr, n)
+
+    def hamiltonian(self, pumping, coeffs, u, op, klu, n=None):
+        """v = hamiltonian(pumping, coeffs, u, op, klu[, n]) -- nls.f90:621 (on the GPU)."""
+        P = _arr(pumping, np.float32, "pumping", "1st")
+        n = P.shape[0] if n is None else int(n)
+        P = _arr(P, np.float32, "pumping", "1st", (n,))
+        co = _arr(coeffs, np.float32, "coeffs", "2nd", (23,))
+        ua = _arr(u, np.complex64, "u", "3rd", (n,))
+        oa = _arr(op, np.float32, "op", "4th", (2 * int(klu) + 1, n), order="F")
+        v = np.empty(n, np.complex64)
+        _check(load().nls_hamiltonian(_ptr(P), _ptr(co), _ptr(ua), _ptr(v), _ptr(oa), int(klu), n))
+        return v
+
+    def hamiltonian_2d(self, pumping, coeffs, u, blocks, orders, order=None, n=None):
+        """v = hamiltonian_2d(pumping, coeffs, u, blocks, orders[, order, n]) -- nls.f90:841 (on the GPU)."""
+        oa = _arr(orders, np.int32, "orders", "5th")
+        order = oa.shape[0] if order is None else int(order)
+        P = _arr(pumping, np.float32, "pumping", "1st")
+        n = P.shape[0] if n is None else int(n)
+        P = _arr(P, np.float32, "pumping", "1st", (n, n))
+        co = _arr(coeffs, np.float32, "coeffs", "2nd", (23,))
+        ua = _arr(u, np.complex64, "u", "3rd", (n, n))
+        ba = _arr(blocks, np.float32, "blocks", "4th", (n, 2 * order - 1), order="F")
+        oa = _arr(oa, np.int32, "orders", "5th", (order,))
+        v = np.empty((n, n), np.complex64)
+        _check(load().nls_hamiltonian_2d(_ptr(P), _ptr(co), _ptr(ua), _ptr(v), _ptr(ba), _ptr(oa), order, n))
+        return v
+
+    def runge_kutta(self, dt, t0, u0, op, iters, pumping, coeffs, n=None, order=None):
+        """u = runge_kutta(dt, t0, u0, op, iters, pumping, coeffs[, n, order]) -- nls.f90:705 (on the GPU)."""
+        ua = _arr(u0, np.complex64, "u0", "3rd")
+        n = ua.shape[0] if n is None else int(n)
+        ua = _arr(ua, np.complex64, "u0", "3rd", (n,))
+        oa = _arr(op, np.float32, "op", "4th", order="F")
+        order = oa.shape[0] if order is None else int(order)
+        oa = _arr(oa, np.float32, "op", "4th", (order, n), order="F")
+        P = _arr(pumping, np.float32, "pumping", "6th", (n,))
+        co = _arr(coeffs, np.float32, "coeffs", "7th", (23,))
+        u = np.empty(n, np.complex64)
+        _check(load().nls_runge_kutta(float(dt), float(t0), _ptr(ua), _ptr(oa), n, order, int(iters), _ptr(u),
+                                      _ptr(P), _ptr(co)))
+        return u
+
+    def runge_kutta_2d(self, dt, t0, u0, blocks, orders, iters, pumping, coeffs, n=None, order=None):
+        """u = runge_kutta_2d(dt, t0, u0, blocks, orders, iters, pumping, coeffs[, n, order]) -- nls.f90:873."""
+        ua = _arr(u0, np.complex64, "u0", "3rd")
+        n = ua.shape[0] if n is None else int(n)
+        ua = _arr(ua, np.complex64, "u0", "3rd", (n, n))
+        oa = _arr(orders, np.int32, "orders", "5th")
+        order = oa.shape[0] if order is None else int(order)
+        oa = _arr(oa, np.int32, "orders", "5th", (order,))
+        ba = _arr(blocks, np.float32, "blocks", "4th", (n, 2 * order - 1), order="F")
+        P = _arr(pumping, np.float32, "pumping", "7th", (n, n))
+        co = _arr(coeffs, np.float32, "coeffs", "8th", (23,))
+        u = np.empty((n, n), np.complex64)
+        _check(load().nls_runge_kutta_2d(float(dt), float(t0), _ptr(ua), n, _ptr(ba), _ptr(oa), order, int(iters),
+                                         _ptr(u), _ptr(P), _ptr(co)))
+        return u
 
 
 nls = _NlsModule()
