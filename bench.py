@@ -370,6 +370,7 @@ def main():
            "config": {"workload": "sweep512: 512 2-D instances, sides 64/128/256 (p=0.5/0.3/0.2), periodic, "
                                   "order 5, dynamic reservoir, dt=5e-4, %d RK4 steps, complex64 state" % K,
                       "instances": len(members), "grid_points": points, "engine": engine,
+                      "step_blocks": "64-step hipGraph replays" if engine != "resident" else "one launch per call",
                       "parallelism": "single GPU"},
            "roofline": roofline}
     if tuning is not None:

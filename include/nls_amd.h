@@ -302,9 +302,16 @@ size_t nls_batch_workspace_bytes(const nls_batch *b);
  * count entries): member i advances min(iters, steps[i]) steps and then
  * holds its state.  Default: unlimited.  Synchronises the stream. */
 int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream);
-/* Advance every member by up to iters steps on device arrays in place. */
+/* Advance every member by up to iters steps on device arrays in place.
+ * Fused and per-stage engines: a call of at least 128 steps in which every
+ * member's budget covers all the steps replays a captured block of 64 steps
+ * as one hipGraph launch (captured at first use on the batch's own stream,
+ * per set of buffers; unless NLS_FLAG_EAGER); the resident engine runs a
+ * block of steps per launch anyway.  A batch is used by one thread at a time. */
 int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, void *nres, void *workspace,
                       size_t workspace_bytes, void *stream);
+/* NLS_FLAG_EAGER: issue every launch directly (no step-block graphs). */
+int nls_batch_set_flags(nls_batch *b, int flags);
 
 #ifdef __cplusplus
 }

@@ -22,7 +22,7 @@ class Batch(object):
     psi0 and n0 (e.g. ``configs.Instance``)."""
 
     def __init__(self, members, dim=2, order=5, bc="periodic", dtype="f32", engine="auto", device=None,
-                 align=64):
+                 align=64, graphs=True):
         import torch
         self.torch = torch
         self.dim, self.order, self.bc, self.dtype = dim, order, bc, dtype
@@ -46,6 +46,8 @@ class Batch(object):
         native._check(native.load().nls_batch_create(dim, order, native.BC[bc], native.DTYPE[dtype],
                                                      native.ENGINE[engine], arr, self.count, C.byref(h)))
         self.handle = h
+        # step-block graphs (advance calls of >= 128 steps that every member's budget covers)
+        native._check(native.load().nls_batch_set_flags(h, 0 if graphs else native.FLAG_EAGER))
         rt = getattr(torch, REAL[dtype])
         ct = getattr(torch, CPLX[dtype])
         npts = max(self.points, int(native.load().nls_batch_points(h)))

@@ -809,7 +809,23 @@ struct nls_batch {
     void *h_steps_pinned = nullptr;
     std::vector<unsigned char> h_mem;   // host image of the member table
     size_t mem_stride;
+    int min_steps = 0x7fffffff;         // smallest member step budget
+    int flags = 0;                      // NLS_FLAG_EAGER: no step-block graphs
+    // the captured step block and the buffers it was captured on
+    hipGraphExec_t graph = nullptr;
+    const void *g_P = nullptr, *g_psi = nullptr, *g_nres = nullptr, *g_ws = nullptr;
+    hipStream_t capture = nullptr;
 };
+
+// Step-block graphs of a batch (fused and per-stage engines): a block of
+// BATCH_GRAPH_BLOCK steps (even: the state ping-pong ends in the caller's
+// buffers) is captured once per set of buffers and replayed as one graph
+// launch.  A captured block bakes in the step index of each launch, which
+// decides where a member whose budget ends inside the call starts passing
+// its state through; blocks are therefore replayed only when every member's
+// budget covers the whole call, and other calls issue their launches
+// directly.
+constexpr int BATCH_GRAPH_BLOCK = 64;
 
 template <typename T>
 static void fill_members(nls_batch *b, const nls_member *ms, long long *tab_total) {
@@ -842,25 +858,16 @@ static void set_member_steps(nls_batch *b, int i, int steps) {
     ((MemberDev<T> *)b->h_mem.data())[i].steps = steps;
 }
 
+// Steps it0 .. it0 + iters - 1 of the fused or per-stage engine, each
+// launch issued on s.
 template <typename T>
-static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws, hipStream_t s) {
+static int batch_steps(nls_batch *b, int it0, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws,
+                       hipStream_t s) {
     using C = cplx_t<T>;
     const size_t cb = align256b(b->points * sizeof(C)), rb = align256b(b->points * sizeof(T));
     const bool dyn = b->dyn;
     C *cur = psi;
     T *curn = nres;
-    if (b->engine == NLS_ENGINE_RESIDENT) {
-        ResidentArgs<T> r;
-        memset(&r, 0, sizeof r);
-        r.mem = (const MemberDev<T> *)b->d_mem;
-        r.iters = iters;
-        r.wtab = (const T *)b->d_wtab;
-        r.psi = psi;
-        r.nres = nres;
-        r.P = P;
-        launch_resident<T>(b->dim, b->order, b->bc, dyn, b->count, r, s);
-        return hip_check(hipGetLastError(), "resident launch");
-    }
     if (b->engine == NLS_ENGINE_FUSED) {
         C *alt = (C *)ws;
         T *altn = dyn ? (T *)(ws + cb) : nullptr;
@@ -872,7 +879,7 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
         f.chunk = b->chunk;
         f.P = P;
         for (int it = 0; it < iters; ++it) {
-            f.it = it;
+            f.it = it0 + it;
             f.u = cur; f.un = curn; f.uo = alt; f.uno = altn;
             launch_fused<T>(b->order, b->bc, dyn, f, s);
             std::swap(cur, alt);
@@ -893,7 +900,7 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
         a.wtab = (const T *)b->d_wtab;
         a.P = P;
         for (int it = 0; it < iters; ++it) {
-            a.it = it;
+            a.it = it0 + it;
             a.u = cur; a.un = curn; a.acc = alt; a.acc_n = altn;
             a.stage = 1; a.scale = 0; a.kin = nullptr; a.kin_n = nullptr; a.kout = kA; a.kout_n = kAn;
             launch_batch_stage<T>(b->dim, b->order, b->bc, dyn, a, s);
@@ -915,6 +922,68 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
             rc = hip_check(hipMemcpyAsync(nres, curn, b->points * sizeof(T), hipMemcpyDeviceToDevice, s), "copy reservoir");
     }
     return rc;
+}
+
+static int drop_graph(nls_batch *b) {
+    if (!b->graph) return NLS_OK;
+    // an earlier launch of it may still be in flight on any stream
+    int rc = hip_check(hipDeviceSynchronize(), "release batch graph");
+    (void)hipGraphExecDestroy(b->graph);
+    b->graph = nullptr;
+    return rc;
+}
+
+// The batch's captured block of BATCH_GRAPH_BLOCK steps on these buffers,
+// captured on the batch's private stream at first use.
+template <typename T>
+static int batch_graph(nls_batch *b, const T *P, cplx_t<T> *psi, T *nres, char *ws) {
+    if (b->graph && b->g_P == P && b->g_psi == psi && b->g_nres == nres && b->g_ws == ws) return NLS_OK;
+    int rc = drop_graph(b);
+    if (rc) return rc;
+    if (!b->capture && (rc = hip_check(hipStreamCreateWithFlags(&b->capture, hipStreamNonBlocking), "capture stream")))
+        return rc;
+    rc = hip_check(hipStreamBeginCapture(b->capture, hipStreamCaptureModeThreadLocal), "begin capture");
+    if (rc) return rc;
+    rc = batch_steps<T>(b, 0, BATCH_GRAPH_BLOCK, P, psi, nres, ws, b->capture);
+    hipGraph_t g = nullptr;
+    const int rc_end = hip_check(hipStreamEndCapture(b->capture, &g), "end capture");
+    if (rc || rc_end) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc ? rc : rc_end;
+    }
+    rc = hip_check(hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0), "instantiate graph");
+    (void)hipGraphDestroy(g);
+    if (rc) {
+        b->graph = nullptr;
+        return rc;
+    }
+    b->g_P = P; b->g_psi = psi; b->g_nres = nres; b->g_ws = ws;
+    return NLS_OK;
+}
+
+template <typename T>
+static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws, hipStream_t s) {
+    if (b->engine == NLS_ENGINE_RESIDENT) {
+        // a persistent launch per block of steps already
+        ResidentArgs<T> r;
+        memset(&r, 0, sizeof r);
+        r.mem = (const MemberDev<T> *)b->d_mem;
+        r.iters = iters;
+        r.wtab = (const T *)b->d_wtab;
+        r.psi = psi;
+        r.nres = nres;
+        r.P = P;
+        launch_resident<T>(b->dim, b->order, b->bc, b->dyn, b->count, r, s);
+        return hip_check(hipGetLastError(), "resident launch");
+    }
+    if ((b->flags & NLS_FLAG_EAGER) || iters < 2 * BATCH_GRAPH_BLOCK || b->min_steps < iters)
+        return batch_steps<T>(b, 0, iters, P, psi, nres, ws, s);
+    int rc = batch_graph<T>(b, P, psi, nres, ws);
+    if (rc) return rc;
+    const int blocks = iters / BATCH_GRAPH_BLOCK, rest = iters % BATCH_GRAPH_BLOCK;
+    for (int k = 0; k < blocks; ++k)
+        if ((rc = hip_check(hipGraphLaunch(b->graph, s), "graph launch"))) return rc;
+    return rest ? batch_steps<T>(b, iters - rest, rest, P, psi, nres, ws, s) : NLS_OK;
 }
 
 extern "C" {
@@ -1016,6 +1085,8 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
 
 int nls_batch_destroy(nls_batch *b) {
     if (!b) return NLS_OK;
+    (void)drop_graph(b);
+    if (b->capture) (void)hipStreamDestroy(b->capture);
     if (b->d_mem) (void)hipFree(b->d_mem);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     if (b->d_wtab) (void)hipFree(b->d_wtab);
@@ -1038,16 +1109,26 @@ size_t nls_batch_workspace_bytes(const nls_batch *b) {
 
 int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream) {
     if (!b || !steps) return fail(NLS_EINVAL, "NULL argument");
-    for (int i = 0; i < b->count; ++i) {
+    for (int i = 0; i < b->count; ++i)
         if (steps[i] < 0) return fail(NLS_EINVAL, "member %d: negative step budget", i);
+    b->min_steps = 0x7fffffff;
+    for (int i = 0; i < b->count; ++i) {
         if (b->dtype == NLS_F64) set_member_steps<double>(b, i, steps[i]);
         else set_member_steps<float>(b, i, steps[i]);
+        if (steps[i] < b->min_steps) b->min_steps = steps[i];
     }
     hipStream_t s = (hipStream_t)stream;
     // the member table may still be read by queued launches: wait, then copy
     int rc = hip_check(hipStreamSynchronize(s), "set_steps sync");
     if (!rc) rc = hip_check(hipMemcpy(b->d_mem, b->h_mem.data(), b->h_mem.size(), hipMemcpyHostToDevice), "set_steps");
     return rc;
+}
+
+int nls_batch_set_flags(nls_batch *b, int flags) {
+    if (!b) return fail(NLS_EINVAL, "NULL batch");
+    if (flags & ~NLS_FLAG_EAGER) return fail(NLS_EINVAL, "unknown flags 0x%x", flags);
+    b->flags = flags;
+    return NLS_OK;
 }
 
 int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, void *nres, void *workspace,

@@ -123,3 +123,59 @@ def test_graph_follows_stage_tile_height(gpu):
     np.testing.assert_array_equal(outs[0], outs[1])
     np.testing.assert_array_equal(outs[0], outs[2])
     native.release()
+
+
+def batch_members(dtype_dim=2):
+    out = []
+    for k, n in enumerate((48, 64, 80)):
+        inst = configs.square(n, 0.2 * n, 2.0 + k, 3.0, 1.0, 30 + k, 1e-3, 0, "b%d" % k)
+        out.append(inst)
+    return out
+
+
+@pytest.mark.parametrize("engine", ("fused", "stage"))
+@pytest.mark.parametrize("dtype", ("f64", "f32"))
+def test_batch_graph_replays_equal_direct_launches(gpu, engine, dtype):
+    """A batch replays a captured 64-step block when every member's budget
+    covers the call: bitwise equal to direct launches, on every replay."""
+    import torch
+    from nls_amd.batch import Batch
+    ms = batch_members()
+    steps = 200                                   # 3 graph blocks + 8 direct steps
+    eager = Batch(ms, dtype=dtype, engine=engine, graphs=False)
+    eager.advance(steps)
+    torch.cuda.synchronize()
+    ref = [eager.member(i) for i in range(len(ms))]
+    g = Batch(ms, dtype=dtype, engine=engine, graphs=True)
+    for rep in range(3):                          # first call captures, later ones replay
+        g.reset()
+        g.advance(steps)
+        g.psi.mul_(1.0)                           # direct work on the same stream between replays
+        torch.cuda.synchronize()
+        for i in range(len(ms)):
+            psi, nres = g.member(i)
+            np.testing.assert_array_equal(psi, ref[i][0])
+            np.testing.assert_array_equal(nres, ref[i][1])
+    eager.close()
+    g.close()
+
+
+@pytest.mark.parametrize("engine", ("fused", "stage"))
+def test_batch_budgets_inside_a_call_are_honoured(gpu, engine):
+    """Budgets that end inside a call: the launches are issued directly, the
+    step index of each is exact, and every member stops at its budget."""
+    import torch
+    from nls_amd.batch import Batch
+    ms = batch_members()
+    budgets = [200, 90, 150]
+    g = Batch(ms, dtype="f32", engine=engine, graphs=True)
+    g.set_steps(budgets)
+    g.advance(200)
+    torch.cuda.synchronize()
+    for i, b in enumerate(budgets):
+        one = Batch([ms[i]], dtype="f32", engine=engine, graphs=False)
+        one.advance(b)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(g.member(i)[0], one.member(0)[0])
+        one.close()
+    g.close()
