@@ -332,6 +332,7 @@ def main():
         batch.advance(W)
     torch.cuda.synchronize()
     batch.reset()
+    batch.prepare(K)                 # capture the 64-step block the timed call replays (runs no step)
     torch.cuda.synchronize()
     log("warmup done (%d steps); timing %d steps" % (W, K))
 
@@ -375,6 +376,13 @@ def main():
            "roofline": roofline}
     if tuning is not None:
         out["engine_tuning_us_per_step"] = tuning
+    # FP64 observables of every member after the timed run (one launch)
+    obs = batch.observables()
+    out["fp64_observables"] = {"members": int(obs.shape[0]), "nonfinite_members": int((obs[:, 3] > 0).sum()),
+                               "norm_min": float(obs[:, 0].min()), "norm_max": float(obs[:, 0].max()),
+                               "population_total": float(obs[:, 1].sum())}
+    log("FP64 observables: %d members, %d with non-finite points, norm %.4g .. %.4g" % (
+        obs.shape[0], out["fp64_observables"]["nonfinite_members"], obs[:, 0].min(), obs[:, 0].max()))
 
     if not args.no_plain:
         # plain FP64 path, one instance at a time, on one instance of each size

@@ -312,6 +312,18 @@ int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, v
                       size_t workspace_bytes, void *stream);
 /* NLS_FLAG_EAGER: issue every launch directly (no step-block graphs). */
 int nls_batch_set_flags(nls_batch *b, int flags);
+/* Capture now the step block that nls_batch_advance(b, iters, ...) on these
+ * buffers would replay (nothing when it would issue its launches directly);
+ * runs no step.  Optional: advance captures at first use otherwise. */
+int nls_batch_prepare(nls_batch *b, int iters, const void *pumping, void *psi, void *nres, void *workspace,
+                      size_t workspace_bytes);
+/* Observables of every member in one launch, accumulated in double
+ * precision in a fixed order (a member's values do not depend on the rest of
+ * the batch; nls_observables of one instance): out[4 i + 0..3] = norm
+ * sum |psi|^2 dV, reservoir population sum n dV, peak |psi|^2 and the count
+ * of non-finite points of member i.  out is a host array of 4 count
+ * doubles; blocks until done. */
+int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, double *out, void *stream);
 
 #ifdef __cplusplus
 }

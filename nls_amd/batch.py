@@ -89,6 +89,26 @@ class Batch(object):
             C.c_void_p(self.nres.data_ptr()) if self.nres is not None else None,
             C.c_void_p(self.ws.data_ptr()), self.ws.numel(), C.c_void_p(s.cuda_stream)))
 
+    def prepare(self, iters):
+        """Capture the step block an advance of `iters` steps would replay
+        (nls_batch_prepare); runs no step."""
+        native._check(native.load().nls_batch_prepare(
+            self.handle, int(iters), C.c_void_p(self.P.data_ptr()), C.c_void_p(self.psi.data_ptr()),
+            C.c_void_p(self.nres.data_ptr()) if self.nres is not None else None,
+            C.c_void_p(self.ws.data_ptr()), self.ws.numel()))
+
+    def observables(self, stream=None):
+        """Per-member (norm, population, peak density, non-finite count),
+        accumulated in double precision on the device (nls_batch_observables):
+        a (count, 4) float64 array."""
+        s = stream if stream is not None else self.torch.cuda.current_stream()
+        out = np.zeros((self.count, 4), np.float64)
+        native._check(native.load().nls_batch_observables(
+            self.handle, C.c_void_p(self.psi.data_ptr()),
+            C.c_void_p(self.nres.data_ptr()) if self.nres is not None else None,
+            out.ctypes.data_as(C.c_void_p), C.c_void_p(s.cuda_stream)))
+        return out
+
     def member(self, i):
         """(psi, n) of member i as host numpy arrays in its grid shape."""
         a, b = self.offsets[i], self.offsets[i] + self.sizes[i]

@@ -809,6 +809,7 @@ struct nls_batch {
     void *h_steps_pinned = nullptr;
     std::vector<unsigned char> h_mem;   // host image of the member table
     size_t mem_stride;
+    std::vector<double> h_dx;           // grid spacing of each member
     int min_steps = 0x7fffffff;         // smallest member step budget
     int flags = 0;                      // NLS_FLAG_EAGER: no step-block graphs
     // the captured step block and the buffers it was captured on
@@ -924,6 +925,51 @@ static int batch_steps(nls_batch *b, int it0, int iters, const T *P, cplx_t<T> *
     return rc;
 }
 
+// Per-member observables: one workgroup per member sums |psi|^2 (times the
+// point index on the 1-D radial grid, whose weight is r = i dx) and the
+// reservoir, takes the peak density and counts non-finite points, in double
+// precision.  Each thread accumulates a fixed strided slice; the wavefront
+// reduces with lane shuffles (xor 32 .. 1) and the four wavefronts' sums are
+// added in order, so a member's values do not depend on the rest of the batch.
+template <typename T>
+__global__ void __launch_bounds__(256) k_batch_obs(const MemberDev<T> *mem, int dim, int radial,
+                                                   const cplx_t<T> *psi, const T *nres, double *out) {
+    __shared__ double part[4][4];
+    const MemberDev<T> m = mem[blockIdx.x];
+    const long long N = dim == 1 ? (long long)m.n : (long long)m.n * m.n;
+    const cplx_t<T> *u = psi + m.off;
+    const T *nn = nres ? nres + m.off : nullptr;
+    double v[4] = {0, 0, 0, 0};
+    for (long long i = threadIdx.x; i < N; i += 256) {
+        const double w = radial ? (double)i : 1.0;
+        const double pr = u[i].x, pi = u[i].y;
+        const double sq = pr * pr + pi * pi;
+        const double r = nn ? (double)nn[i] : 0.0;
+        if (!isfinite(sq) || !isfinite(r)) { v[3] += 1.0; continue; }
+        v[0] += sq * w;
+        v[1] += r * w;
+        v[2] = sq > v[2] ? sq : v[2];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        v[0] += __shfl_xor(v[0], off, 64);
+        v[1] += __shfl_xor(v[1], off, 64);
+        v[2] = fmax(v[2], __shfl_xor(v[2], off, 64));
+        v[3] += __shfl_xor(v[3], off, 64);
+    }
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 4; ++q) part[wave][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[4] = {part[0][0], part[0][1], part[0][2], part[0][3]};
+        for (int w = 1; w < 4; ++w) {
+            o[0] += part[w][0]; o[1] += part[w][1]; o[2] = fmax(o[2], part[w][2]); o[3] += part[w][3];
+        }
+        for (int q = 0; q < 4; ++q) out[(size_t)blockIdx.x * 4 + q] = o[q];
+    }
+}
+
 static int drop_graph(nls_batch *b) {
     if (!b->graph) return NLS_OK;
     // an earlier launch of it may still be in flight on any stream
@@ -961,6 +1007,11 @@ static int batch_graph(nls_batch *b, const T *P, cplx_t<T> *psi, T *nres, char *
     return NLS_OK;
 }
 
+static bool batch_graphs_apply(const nls_batch *b, int iters) {
+    return b->engine != NLS_ENGINE_RESIDENT && !(b->flags & NLS_FLAG_EAGER) && iters >= 2 * BATCH_GRAPH_BLOCK &&
+           b->min_steps >= iters;
+}
+
 template <typename T>
 static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws, hipStream_t s) {
     if (b->engine == NLS_ENGINE_RESIDENT) {
@@ -976,8 +1027,7 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
         launch_resident<T>(b->dim, b->order, b->bc, b->dyn, b->count, r, s);
         return hip_check(hipGetLastError(), "resident launch");
     }
-    if ((b->flags & NLS_FLAG_EAGER) || iters < 2 * BATCH_GRAPH_BLOCK || b->min_steps < iters)
-        return batch_steps<T>(b, 0, iters, P, psi, nres, ws, s);
+    if (!batch_graphs_apply(b, iters)) return batch_steps<T>(b, 0, iters, P, psi, nres, ws, s);
     int rc = batch_graph<T>(b, P, psi, nres, ws);
     if (rc) return rc;
     const int blocks = iters / BATCH_GRAPH_BLOCK, rest = iters % BATCH_GRAPH_BLOCK;
@@ -1039,6 +1089,7 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     nls_batch *b = new nls_batch();
     b->dim = dim; b->order = order; b->bc = bc; b->dtype = dtype; b->engine = engine; b->dyn = dyn;
     b->count = count; b->points = points;
+    for (int i = 0; i < count; ++i) b->h_dx.push_back(members[i].dx);
     long long tab_total = 0;
     if (dtype == NLS_F64) fill_members<double>(b, members, &tab_total);
     else fill_members<float>(b, members, &tab_total);
@@ -1122,6 +1173,51 @@ int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream) {
     int rc = hip_check(hipStreamSynchronize(s), "set_steps sync");
     if (!rc) rc = hip_check(hipMemcpy(b->d_mem, b->h_mem.data(), b->h_mem.size(), hipMemcpyHostToDevice), "set_steps");
     return rc;
+}
+
+int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, double *out, void *stream) {
+    if (!b || !psi || !out) return fail(NLS_EINVAL, "NULL argument");
+    hipStream_t s = (hipStream_t)stream;
+    const int radial = (b->dim == 1 && b->bc == NLS_BC_REF) ? 1 : 0;
+    void *d_out = nullptr;
+    int rc = hip_check(hipMalloc(&d_out, (size_t)b->count * 4 * sizeof(double)), "observables buffer");
+    if (rc) return rc;
+    if (b->dtype == NLS_F64)
+        hipLaunchKernelGGL((k_batch_obs<double>), dim3(b->count), dim3(256), 0, s, (const MemberDev<double> *)b->d_mem,
+                           b->dim, radial, (const double2 *)psi, b->dyn ? (const double *)nres : nullptr,
+                           (double *)d_out);
+    else
+        hipLaunchKernelGGL((k_batch_obs<float>), dim3(b->count), dim3(256), 0, s, (const MemberDev<float> *)b->d_mem,
+                           b->dim, radial, (const float2 *)psi, b->dyn ? (const float *)nres : nullptr,
+                           (double *)d_out);
+    rc = hip_check(hipGetLastError(), "observables launch");
+    if (!rc) rc = hip_check(hipMemcpyAsync(out, d_out, (size_t)b->count * 4 * sizeof(double), hipMemcpyDeviceToHost, s),
+                            "observables download");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "observables sync");
+    (void)hipFree(d_out);
+    if (rc) return rc;
+    for (int i = 0; i < b->count; ++i) {
+        // volume element, as nls_observables: 2 pi r dr on the radial grid
+        // (the kernel summed with weight i, r = i dx), dx on a line, dx^2 on a square
+        const double dx = b->h_dx[i];
+        const double dV = radial ? 2.0 * M_PI * dx * dx : (b->dim == 1 ? dx : dx * dx);
+        out[4 * i + 0] *= dV;
+        out[4 * i + 1] *= dV;
+    }
+    return NLS_OK;
+}
+
+int nls_batch_prepare(nls_batch *b, int iters, const void *pumping, void *psi, void *nres, void *workspace,
+                      size_t workspace_bytes) {
+    if (!b) return fail(NLS_EINVAL, "NULL batch");
+    if (!pumping || !psi || (b->dyn && !nres)) return fail(NLS_EINVAL, "NULL device buffer");
+    const size_t need = nls_batch_workspace_bytes(b);
+    if (workspace_bytes < need || (need && !workspace))
+        return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+    if (!batch_graphs_apply(b, iters)) return NLS_OK;
+    if (b->dtype == NLS_F64)
+        return batch_graph<double>(b, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)workspace);
+    return batch_graph<float>(b, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)workspace);
 }
 
 int nls_batch_set_flags(nls_batch *b, int flags) {

@@ -147,7 +147,8 @@ def test_batch_graph_replays_equal_direct_launches(gpu, engine, dtype):
     torch.cuda.synchronize()
     ref = [eager.member(i) for i in range(len(ms))]
     g = Batch(ms, dtype=dtype, engine=engine, graphs=True)
-    for rep in range(3):                          # first call captures, later ones replay
+    g.prepare(steps)                              # captures the block now; every call below replays it
+    for rep in range(3):
         g.reset()
         g.advance(steps)
         g.psi.mul_(1.0)                           # direct work on the same stream between replays
