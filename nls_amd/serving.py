@@ -189,11 +189,21 @@ class Group(object):
             r.remaining -= int(b)
 
     def retire(self, now_fn):
-        """Download and free the members that have finished."""
+        """Download and free the members that have finished, with their FP64
+        observables (norm, population, peak density, non-finite count)."""
+        import ctypes as C
         done = [r for r in self.members if r.remaining == 0]
         if not done:
             return []
-        self.stream.synchronize()
+        obs = np.zeros((len(self.members), 4), np.float64)      # one launch for the whole pool
+        native._check(native.load().nls_batch_observables(
+            self.handle, C.c_void_p(self.psi.data_ptr()), C.c_void_p(self.nres.data_ptr()),
+            obs.ctypes.data_as(C.c_void_p), C.c_void_p(self.stream.cuda_stream)))
+        index = {id(r): i for i, r in enumerate(self.members)}
+        for r in done:
+            o = obs[index[id(r)]]
+            r.observables = {"norm": float(o[0]), "population": float(o[1]), "peak": float(o[2]),
+                             "nonfinite": int(o[3])}
         for r in done:
             a, b = r.offset, r.offset + r.points
             shape = (r.n,) if r.dim == 1 else (r.n, r.n)

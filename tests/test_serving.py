@@ -73,3 +73,10 @@ def test_result_alone_equals_result_in_the_mix(gpu, oracle):
     opsi, _ = oracle.rk4(r.dim, r.n, 5, 1, r.dt, r.dx, r.steps, r.P, r.coeffs, r.psi0, r.n0)
     assert np.linalg.norm(r.psi - opsi) / np.linalg.norm(opsi) <= 1e-4
     assert all(r.latency > 0 for r in done)
+    # every result carries its FP64 observables, from the state it returns
+    for r in done:
+        dV = r.dx if r.dim == 1 else r.dx * r.dx
+        s2 = np.abs(r.psi.astype(np.complex128)) ** 2
+        assert r.observables["nonfinite"] == 0
+        assert r.observables["norm"] == pytest.approx(s2.sum() * dV, rel=1e-12)
+        assert r.observables["population"] == pytest.approx(r.nres.astype(np.float64).sum() * dV, rel=1e-12)
