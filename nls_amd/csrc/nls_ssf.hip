@@ -402,15 +402,21 @@ __global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<T>
     __shared__ C buf[LINE_PTS];
     const int logN = a.logN, N = 1 << logN;
     T pp[PPT], nn[PPT];
+    C ph[PPT];                                       // the phase of each point, for every step
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int idx = threadIdx.x + LINE_NTH * q;
-        pp[q] = 0; nn[q] = 0;
+        pp[q] = 0; nn[q] = 0; ph[q] = mk<T>(0, 0);
         if (idx < N) {
-            buf[idx] = a.psi[idx];
+            ph[q] = a.ph[idx];
             pp[q] = a.P[idx];
             if (DYN) nn[q] = a.nres[idx];
         }
+    }
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + LINE_NTH * q;
+        if (idx < N) buf[idx] = a.psi[idx];
     }
     for (int it = 0; it < a.iters; ++it) {
         // each thread only touches its own points here, so no barrier is
@@ -429,7 +435,7 @@ __global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<T>
 #pragma unroll
         for (int q = 0; q < PPT; ++q) {
             const int idx = threadIdx.x + LINE_NTH * q;
-            if (idx < N) buf[idx] = cmul<T>(buf[idx], a.ph[idx]);
+            if (idx < N) buf[idx] = cmul<T>(buf[idx], ph[q]);
         }
         __syncthreads();
         lds_fft<T, true, false, LINE_PTS, LINE_NTH>(buf, logN, 0, N, 1, a.tw);
