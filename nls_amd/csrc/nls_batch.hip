@@ -522,8 +522,16 @@ template <typename T, int ORDER, int BC, bool DYN, int DIM>
 __global__ void __launch_bounds__(ResShape<T>::THREADS) k_resident(const ResidentArgs<T> r) {
     using C = cplx_t<T>;
     constexpr int RES_THREADS = ResShape<T>::THREADS, RES_PPT = RES_MAXP / RES_THREADS;
-    __shared__ C U[RES_MAXP];
-    __shared__ C Y[RES_MAXP];
+    // Single precision: Y[b] is the psi stage input, double-buffered -- stage
+    // s reads Y[b] and writes the next stage's input into Y[b ^ 1], so one
+    // barrier per stage suffices (a thread writes Y[b] again only after the
+    // next barrier, which every thread reaches after its reads of Y[b]); the
+    // state at the start of the step is read at the thread's own points only
+    // and stays in registers.  Double precision has no registers to spare for
+    // that state (it would spill): Y[0] is the stage input, Y[1] the state,
+    // and each stage waits for all reads of Y[0] before writing it.
+    constexpr bool PINGPONG = sizeof(T) == 4;
+    __shared__ C Y[2][RES_MAXP];
     const MemberDev<T> m = r.mem[blockIdx.x];
     const int n = m.n;
     const int N = DIM == 1 ? n : n * n;
@@ -537,23 +545,34 @@ __global__ void __launch_bounds__(ResShape<T>::THREADS) k_resident(const Residen
     const T dt = m.dt;
     const int tid = threadIdx.x;
     T pp[RES_PPT], un[RES_PPT], yn[RES_PPT], accn[RES_PPT];
-    C acc[RES_PPT];
+    C acc[RES_PPT], uu[PINGPONG ? RES_PPT : 1];
 #pragma unroll
     for (int q = 0; q < RES_PPT; ++q) {
         const int idx = tid + RES_THREADS * q;
         pp[q] = 0; un[q] = 0; yn[q] = 0; accn[q] = 0; acc[q] = mk<T>(0, 0);
+        if (PINGPONG) uu[PINGPONG ? q : 0] = mk<T>(0, 0);
         if (idx < N) {
             const C v = psi[idx];
-            U[idx] = v;
-            Y[idx] = v;
+            if (PINGPONG) uu[PINGPONG ? q : 0] = v;
+            else Y[1][idx] = v;
             pp[q] = P[idx];
             if (DYN) { un[q] = nres[idx]; yn[q] = un[q]; }
         }
     }
+#pragma unroll
+    for (int q = 0; q < RES_PPT; ++q) {
+        const int idx = tid + RES_THREADS * q;
+        if (idx < N) Y[0][idx] = PINGPONG ? uu[PINGPONG ? q : 0] : Y[1][idx];
+    }
     __syncthreads();
+    int b = 0;
     for (int it = 0; it < steps; ++it) {
 #pragma unroll
         for (int s = 1; s <= 4; ++s) {
+            const C *Yin = Y[b];
+            C *Yout = Y[PINGPONG ? (b ^ 1) : 0];
+            const T wacc = (s == 1 || s == 4) ? (T)1 : (T)2;
+            const T anext = (s == 3) ? (T)1 : (T)0.5;
             C kk[RES_PPT];
             T kn[RES_PPT];
 #pragma unroll
@@ -561,13 +580,11 @@ __global__ void __launch_bounds__(ResShape<T>::THREADS) k_resident(const Residen
                 const int idx = tid + RES_THREADS * q;
                 kk[q] = mk<T>(0, 0); kn[q] = 0;
                 if (idx < N) {
-                    const C lap = resident_lap<T, ORDER, BC, DIM>(Y, idx, n, w, wtab ? wtab + (size_t)idx * ORDER : nullptr);
-                    rhs_point<T, DYN>(m.c, pp[q], Y[idx], yn[q], lap, kk[q], kn[q]);
+                    const C lap = resident_lap<T, ORDER, BC, DIM>(Yin, idx, n, w, wtab ? wtab + (size_t)idx * ORDER : nullptr);
+                    rhs_point<T, DYN>(m.c, pp[q], Yin[idx], yn[q], lap, kk[q], kn[q]);
                 }
             }
-            __syncthreads();                         // every read of Y for this stage is done
-            const T wacc = (s == 1 || s == 4) ? (T)1 : (T)2;
-            const T anext = (s == 3) ? (T)1 : (T)0.5;
+            if (!PINGPONG) __syncthreads();          // every read of Y[0] for this stage is done
 #pragma unroll
             for (int q = 0; q < RES_PPT; ++q) {
                 const int idx = tid + RES_THREADS * q;
@@ -577,25 +594,27 @@ __global__ void __launch_bounds__(ResShape<T>::THREADS) k_resident(const Residen
                     acc[q] = mk<T>(acc[q].x + wacc * kk[q].x, acc[q].y + wacc * kk[q].y);
                     accn[q] = accn[q] + wacc * kn[q];
                 }
-                const C u = U[idx];
+                const C u = PINGPONG ? uu[PINGPONG ? q : 0] : Y[1][idx];
                 if (s < 4) {
-                    Y[idx] = mk<T>(u.x + (kk[q].x * dt) * anext, u.y + (kk[q].y * dt) * anext);
+                    Yout[idx] = mk<T>(u.x + (kk[q].x * dt) * anext, u.y + (kk[q].y * dt) * anext);
                     if (DYN) yn[q] = un[q] + (kn[q] * dt) * anext;
                 } else {
                     const C v = mk<T>(u.x + (acc[q].x * dt) / (T)6, u.y + (acc[q].y * dt) / (T)6);
-                    U[idx] = v;
-                    Y[idx] = v;
+                    if (PINGPONG) uu[PINGPONG ? q : 0] = v;
+                    else Y[1][idx] = v;
+                    Yout[idx] = v;
                     if (DYN) { un[q] = un[q] + (accn[q] * dt) / (T)6; yn[q] = un[q]; }
                 }
             }
             __syncthreads();
+            if (PINGPONG) b ^= 1;
         }
     }
 #pragma unroll
     for (int q = 0; q < RES_PPT; ++q) {
         const int idx = tid + RES_THREADS * q;
         if (idx < N) {
-            psi[idx] = U[idx];
+            psi[idx] = PINGPONG ? uu[PINGPONG ? q : 0] : Y[1][idx];
             if (DYN) nres[idx] = un[q];
         }
     }
