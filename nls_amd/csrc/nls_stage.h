@@ -155,26 +155,49 @@ __device__ __forceinline__ void stage_tile_2d(const StageArgs<T> &a, int x0, int
         acc[j] = (own[j] && a.stage >= 3) ? a.acc[gi[j]] : mk<T>(0, 0);
         accn[j] = (DYN && own[j] && a.stage >= 3) ? a.acc_n[gi[j]] : (T)0;
     }
-    // halo ring
-    for (int h = tid; h < HALO; h += 256) {
+    // halo ring: every halo point's loads of this thread are issued before
+    // any LDS store, so the thread waits out the memory latency once
+    constexpr int HITER = (HALO + 255) / 256;
+    C hv[HITER], hk[HITER];
+    int hl[HITER];
+#pragma unroll
+    for (int q0 = 0; q0 < HITER; ++q0) {
+        const int h = tid + 256 * q0;
+        const bool mine = h < HALO;
+        const int hh = mine ? h : 0;
         int lx, ly;
-        if (h < 2 * R * TILE2_X) {                  // rows above and below the tile
-            const int r = h / TILE2_X;
-            lx = R + (h - r * TILE2_X);
+        if (hh < 2 * R * TILE2_X) {                 // rows above and below the tile
+            const int r = hh / TILE2_X;
+            lx = R + (hh - r * TILE2_X);
             ly = r < R ? r : TY + r;                 // r in [0,R): top; [R,2R): bottom
         } else {                                     // columns left and right, full height
-            const int q = h - 2 * R * TILE2_X;
+            const int q = hh - 2 * R * TILE2_X;
             const int c = q / LH;
             ly = q - c * LH;
             lx = c < R ? c : TILE2_X + c;            // c in [0,R): left; [R,2R): right
         }
         int hx = x0 + lx - R, hy = y0 + ly - R;
-        C y = mk<T>(0, 0);
-        bool ok = true;
+        bool ok = mine;
         if (BC == BC_PERIODIC) { hx = wrap_index(hx, n); hy = wrap_index(hy, n); }
-        else ok = hx >= 0 && hx < n && hy >= 0 && hy < n;
-        if (ok) y = stage_input(a, (size_t)hy * n + hx);
-        tile[ly * LW + lx] = y;
+        else ok = ok && hx >= 0 && hx < n && hy >= 0 && hy < n;
+        // unconditional loads (index 0 when the point is not used) keep every
+        // halo load of the thread in one group; the stage input is formed at
+        // the store, as stage_input does
+        const size_t hi = ok ? (size_t)hy * n + hx : (size_t)0;
+        const C v = a.u[hi];
+        const C k = first ? mk<T>(0, 0) : a.kin[hi];
+        hv[q0] = ok ? v : mk<T>(0, 0);
+        hk[q0] = ok ? k : mk<T>(0, 0);
+        hl[q0] = mine ? ly * LW + lx : -1;
+    }
+#pragma unroll
+    for (int q0 = 0; q0 < HITER; ++q0) {
+        C y = hv[q0];
+        if (!first) {
+            y.x = y.x + (hk[q0].x * a.dt) * a.scale;
+            y.y = y.y + (hk[q0].y * a.dt) * a.scale;
+        }
+        if (hl[q0] >= 0) tile[hl[q0]] = y;
     }
     // interior: the own points' stage inputs
 #pragma unroll
