@@ -527,10 +527,11 @@ __global__ void __launch_bounds__(ResShape<T>::THREADS) k_resident(const Residen
     // barrier per stage suffices (a thread writes Y[b] again only after the
     // next barrier, which every thread reaches after its reads of Y[b]); the
     // state at the start of the step is read at the thread's own points only
-    // and stays in registers.  Double precision has no registers to spare for
-    // that state (it would spill): Y[0] is the stage input, Y[1] the state,
-    // and each stage waits for all reads of Y[0] before writing it.
-    constexpr bool PINGPONG = sizeof(T) == 4;
+    // and stays in registers.  Double precision does the same on 1-D lines of
+    // order 3 and 5; elsewhere it has no registers to spare for that state (it
+    // would spill): Y[0] is the stage input, Y[1] the state, and each stage
+    // waits for all reads of Y[0] before writing it.
+    constexpr bool PINGPONG = sizeof(T) == 4 || (DIM == 1 && ORDER < 7);
     __shared__ C Y[2][RES_MAXP];
     const MemberDev<T> m = r.mem[blockIdx.x];
     const int n = m.n;
@@ -569,45 +570,61 @@ __global__ void __launch_bounds__(ResShape<T>::THREADS) k_resident(const Residen
     for (int it = 0; it < steps; ++it) {
 #pragma unroll
         for (int s = 1; s <= 4; ++s) {
-            const C *Yin = Y[b];
-            C *Yout = Y[PINGPONG ? (b ^ 1) : 0];
             const T wacc = (s == 1 || s == 4) ? (T)1 : (T)2;
             const T anext = (s == 3) ? (T)1 : (T)0.5;
-            C kk[RES_PPT];
-            T kn[RES_PPT];
-#pragma unroll
-            for (int q = 0; q < RES_PPT; ++q) {
-                const int idx = tid + RES_THREADS * q;
-                kk[q] = mk<T>(0, 0); kn[q] = 0;
-                if (idx < N) {
-                    const C lap = resident_lap<T, ORDER, BC, DIM>(Yin, idx, n, w, wtab ? wtab + (size_t)idx * ORDER : nullptr);
-                    rhs_point<T, DYN>(m.c, pp[q], Yin[idx], yn[q], lap, kk[q], kn[q]);
-                }
-            }
-            if (!PINGPONG) __syncthreads();          // every read of Y[0] for this stage is done
-#pragma unroll
-            for (int q = 0; q < RES_PPT; ++q) {
-                const int idx = tid + RES_THREADS * q;
-                if (idx >= N) continue;
-                if (s == 1) { acc[q] = kk[q]; accn[q] = kn[q]; }
+            // acc += wacc k; the next stage input u + k dt anext, or at stage
+            // 4 the new state u + acc dt / 6
+            auto advance_point = [&](int q, int idx, const C &kk, T kn, const C &u, C *Yout) {
+                if (s == 1) { acc[q] = kk; accn[q] = kn; }
                 else {
-                    acc[q] = mk<T>(acc[q].x + wacc * kk[q].x, acc[q].y + wacc * kk[q].y);
-                    accn[q] = accn[q] + wacc * kn[q];
+                    acc[q] = mk<T>(acc[q].x + wacc * kk.x, acc[q].y + wacc * kk.y);
+                    accn[q] = accn[q] + wacc * kn;
                 }
-                const C u = PINGPONG ? uu[PINGPONG ? q : 0] : Y[1][idx];
                 if (s < 4) {
-                    Yout[idx] = mk<T>(u.x + (kk[q].x * dt) * anext, u.y + (kk[q].y * dt) * anext);
-                    if (DYN) yn[q] = un[q] + (kn[q] * dt) * anext;
-                } else {
-                    const C v = mk<T>(u.x + (acc[q].x * dt) / (T)6, u.y + (acc[q].y * dt) / (T)6);
-                    if (PINGPONG) uu[PINGPONG ? q : 0] = v;
-                    else Y[1][idx] = v;
-                    Yout[idx] = v;
-                    if (DYN) { un[q] = un[q] + (accn[q] * dt) / (T)6; yn[q] = un[q]; }
+                    Yout[idx] = mk<T>(u.x + (kk.x * dt) * anext, u.y + (kk.y * dt) * anext);
+                    if (DYN) yn[q] = un[q] + (kn * dt) * anext;
+                    return u;
                 }
+                const C v = mk<T>(u.x + (acc[q].x * dt) / (T)6, u.y + (acc[q].y * dt) / (T)6);
+                Yout[idx] = v;
+                if (DYN) { un[q] = un[q] + (accn[q] * dt) / (T)6; yn[q] = un[q]; }
+                return v;
+            };
+            if constexpr (PINGPONG) {
+                const C *Yin = Y[b];
+                C *Yout = Y[b ^ 1];
+#pragma unroll
+                for (int q = 0; q < RES_PPT; ++q) {
+                    const int idx = tid + RES_THREADS * q;
+                    if (idx >= N) continue;
+                    C kk;
+                    T kn = 0;
+                    const C lap = resident_lap<T, ORDER, BC, DIM>(Yin, idx, n, w, wtab ? wtab + (size_t)idx * ORDER : nullptr);
+                    rhs_point<T, DYN>(m.c, pp[q], Yin[idx], yn[q], lap, kk, kn);
+                    uu[PINGPONG ? q : 0] = advance_point(q, idx, kk, kn, uu[PINGPONG ? q : 0], Yout);
+                }
+                __syncthreads();
+                b ^= 1;
+            } else {
+                C kk[RES_PPT];
+                T kn[RES_PPT];
+#pragma unroll
+                for (int q = 0; q < RES_PPT; ++q) {
+                    const int idx = tid + RES_THREADS * q;
+                    kk[q] = mk<T>(0, 0); kn[q] = 0;
+                    if (idx < N) {
+                        const C lap = resident_lap<T, ORDER, BC, DIM>(Y[0], idx, n, w, wtab ? wtab + (size_t)idx * ORDER : nullptr);
+                        rhs_point<T, DYN>(m.c, pp[q], Y[0][idx], yn[q], lap, kk[q], kn[q]);
+                    }
+                }
+                __syncthreads();                     // every read of Y[0] for this stage is done
+#pragma unroll
+                for (int q = 0; q < RES_PPT; ++q) {
+                    const int idx = tid + RES_THREADS * q;
+                    if (idx < N) Y[1][idx] = advance_point(q, idx, kk[q], kn[q], Y[1][idx], Y[0]);
+                }
+                __syncthreads();
             }
-            __syncthreads();
-            if (PINGPONG) b ^= 1;
         }
     }
 #pragma unroll
