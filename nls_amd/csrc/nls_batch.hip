@@ -18,6 +18,7 @@
 //                     from HBM plus the halo re-reads, at the price of
 //                     recomputing the halo's stages.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <vector>
 
@@ -157,16 +158,21 @@ template <typename T> struct FusedArgs {
 // double-buffered in LDS (stage s reads one buffer and writes the next
 // stage's input into the other: one barrier per stage and no registers
 // holding the next input).  WAVES: waves per SIMD the register budget is
-// sized for.
-template <typename T, int ORDER> struct FusedTile;
-template <int ORDER> struct FusedTile<float, ORDER> {
-    static constexpr int TY = ORDER == 7 ? 16 : 32;
+// sized for.  SHORT: 16-row tiles at every order, for launches with no more
+// tiles than the device has compute units (one lone grid, a few small
+// members): the step time is then one workgroup's critical path, which grows
+// with the region points per thread (2.5 instead of 3.75 at orders 3 and 5).
+// Launches of more tiles use the taller tiles, whose halo costs less per
+// output point.
+template <typename T, int ORDER, bool SHORT = false> struct FusedTile;
+template <int ORDER, bool SHORT> struct FusedTile<float, ORDER, SHORT> {
+    static constexpr int TY = ORDER == 7 || SHORT ? 16 : 32;
     static constexpr int NT = 1024;
     static constexpr bool DB = true;
     static constexpr int WAVES = 8;
 };
-template <int ORDER> struct FusedTile<double, ORDER> {
-    static constexpr int TY = ORDER == 7 ? 16 : 32;
+template <int ORDER, bool SHORT> struct FusedTile<double, ORDER, SHORT> {
+    static constexpr int TY = ORDER == 7 || SHORT ? 16 : 32;
     static constexpr int NT = 1024;
     static constexpr bool DB = true;
     static constexpr int WAVES = 4;
@@ -175,9 +181,9 @@ template <int ORDER> struct FusedTile<double, ORDER> {
 // ring order's side bands (2R points per row over consecutive rows) fall in
 // distinct banks.
 constexpr int fused_lds_stride(int W, int R) { return W + (R == 1 ? 2 : R == 2 ? 4 : 14); }
-static int fused_ty(int dtype, int order) {
-    return dtype == NLS_F64 ? (order == 7 ? FusedTile<double, 7>::TY : FusedTile<double, 5>::TY)
-                            : (order == 7 ? FusedTile<float, 7>::TY : FusedTile<float, 5>::TY);
+static int fused_ty(int dtype, int order, bool shrt) {
+    if (shrt || order == 7) return 16;
+    return dtype == NLS_F64 ? FusedTile<double, 5>::TY : FusedTile<float, 5>::TY;
 }
 
 // Region points in ring order, built at compile time: entry k holds the
@@ -198,13 +204,14 @@ template <int W, int HH, int R, int WS> struct RingMap {
 };
 static_assert(RingMap<80, 48, 2, 84>().v[0] == (8 * 84 + 8 | 8 << 16 | 8 << 24), "ring order starts at the output tile");
 static_assert(RingMap<80, 48, 2, 84>().v[2048] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the 64 x 32 tile");
+static_assert(RingMap<80, 32, 2, 84>().v[1024] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the 64 x 16 tile");
 template <int W, int HH, int R, int WS> __device__ const RingMap<W, HH, R, WS> ring_map{};
 
-template <typename T, int ORDER, int BC, bool DYN>
-__global__ void __launch_bounds__((FusedTile<T, ORDER>::NT), (FusedTile<T, ORDER>::WAVES))
+template <typename T, int ORDER, int BC, bool DYN, bool SHORT>
+__global__ void __launch_bounds__((FusedTile<T, ORDER, SHORT>::NT), (FusedTile<T, ORDER, SHORT>::WAVES))
 k_fused_step_2d(const FusedArgs<T> f) {
     using C = cplx_t<T>;
-    using FT = FusedTile<T, ORDER>;
+    using FT = FusedTile<T, ORDER, SHORT>;
     constexpr int R = (ORDER - 1) / 2, H = 4 * R;
     constexpr int TX = 64, TY = FT::TY, NT = FT::NT;
     constexpr bool DB = FT::DB;
@@ -658,18 +665,52 @@ static void launch_batch_stage(int dim, int order, int bc, bool dyn, const Batch
 #undef NLS_CASE
 }
 
+// shrt: the 16-row tiles of orders 3 and 5 (order 7 has only 16-row tiles).
 template <typename T>
-static void launch_fused(int order, int bc, bool dyn, const FusedArgs<T> &f, hipStream_t s) {
+static void launch_fused(int order, int bc, bool dyn, bool shrt, const FusedArgs<T> &f, hipStream_t s) {
     const dim3 grid(8 * f.chunk);
-#define NLS_CASE(O, B, D) \
-    if (order == O && bc == B && dyn == D) { \
-        hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D>), grid, dim3(FusedTile<T, O>::NT), 0, s, f); return; }
+    shrt = shrt && order != 7;
+#define NLS_CASE(O, B, D)                                                                                       \
+    if (order == O && bc == B && dyn == D) {                                                                   \
+        if (shrt)                                                                                              \
+            hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, O != 7>), grid, dim3(FusedTile<T, O, true>::NT), 0, \
+                               s, f);                                                                          \
+        else                                                                                                   \
+            hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, false>), grid, dim3(FusedTile<T, O>::NT), 0, s, f); \
+        return;                                                                                                \
+    }
     NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
     NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
     NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
     NLS_CASE(3, 2, false) NLS_CASE(3, 2, true) NLS_CASE(5, 2, false) NLS_CASE(5, 2, true)
     NLS_CASE(7, 2, false) NLS_CASE(7, 2, true)
 #undef NLS_CASE
+}
+
+// Compute units of the current device, read once per device.
+static int device_cus() {
+    static std::atomic<int> cache[64];
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 0;
+    int v = cache[dev].load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    cache[dev].store(v, std::memory_order_relaxed);
+    return v;
+}
+
+// Whether a fused launch over `tiles16` 16-row tiles runs them short: when
+// each gets a compute unit of its own, the step takes one short workgroup's
+// time (measured on MI355X, order 5: a lone 256^2 grid 7.1 -> 5.2 us per step
+// in single precision, 8.8 -> 6.2 us in double; 512^2, 256 short tiles,
+// 7.3 -> 5.7 us and 9.4 -> 7.1 us).  Where compute units would hold two
+// short tiles the taller tiles are as fast or faster (a 2048 x 256 row slab:
+// 9.7 us short, 9.5 us tall), and launches of several rounds are faster tall
+// (a batch of 512^2-class grids: 94.6 us tall, 112.4 us short).
+static bool fused_short_for(int order, long long tiles16) {
+    if (order == 7) return false;
+    const int cus = device_cus();
+    return cus > 0 && tiles16 <= cus;
 }
 
 // The double-precision 2-D order-7 variant would spill registers to scratch;
@@ -736,7 +777,8 @@ static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T>
     using C = cplx_t<T>;
     const size_t N = (size_t)p->n * p->n;
     const bool dyn = p->coeffs[10] != 0.0;
-    const int ty = fused_ty(p->dtype, p->order);
+    const bool shrt = fused_short_for(p->order, (long long)((p->n + 63) / 64) * ((p->n + 15) / 16));
+    const int ty = fused_ty(p->dtype, p->order, shrt);
     FusedArgs<T> f;
     memset(&f, 0, sizeof f);
     f.grid_ntx = (p->n + 63) / 64;
@@ -749,7 +791,7 @@ static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T>
     for (int it = 0; it < iters; ++it) {
         f.it = it;
         f.u = cur; f.un = curn; f.uo = alt; f.uno = altn;
-        launch_fused<T>(p->order, p->bc, dyn, f, s);
+        launch_fused<T>(p->order, p->bc, dyn, shrt, f, s);
         std::swap(cur, alt);
         std::swap(curn, altn);
     }
@@ -772,7 +814,8 @@ template <typename T>
 static int fused_slab_run(const nls_problem *p, int rows, const T *P, const cplx_t<T> *psi_in, const T *nres_in,
                           cplx_t<T> *psi_out, T *nres_out, hipStream_t s) {
     const int H = 2 * (p->order - 1);
-    const int ty = fused_ty(p->dtype, p->order);
+    const bool shrt = fused_short_for(p->order, (long long)((p->n + 63) / 64) * ((rows + 15) / 16));
+    const int ty = fused_ty(p->dtype, p->order, shrt);
     FusedArgs<T> f;
     memset(&f, 0, sizeof f);
     f.grid_ntx = (p->n + 63) / 64;
@@ -782,7 +825,7 @@ static int fused_slab_run(const nls_problem *p, int rows, const T *P, const cplx
     f.slab_end = H + rows;
     f.P = P;
     f.u = psi_in; f.un = nres_in; f.uo = psi_out; f.uno = nres_out;
-    launch_fused<T>(p->order, BC_SLAB, p->coeffs[10] != 0.0, f, s);
+    launch_fused<T>(p->order, BC_SLAB, p->coeffs[10] != 0.0, shrt, f, s);
     return hip_check(hipGetLastError(), "slab step launch");
 }
 
@@ -871,6 +914,7 @@ struct nls_batch {
     size_t mem_stride;
     std::vector<double> h_dx;           // grid spacing of each member
     int min_steps = 0x7fffffff;         // smallest member step budget
+    bool fshort = false;                // fused engine: 16-row tiles
     int flags = 0;                      // NLS_FLAG_EAGER: no step-block graphs
     // the captured step block and the buffers it was captured on
     hipGraphExec_t graph = nullptr;
@@ -942,7 +986,7 @@ static int batch_steps(nls_batch *b, int it0, int iters, const T *P, cplx_t<T> *
         for (int it = 0; it < iters; ++it) {
             f.it = it0 + it;
             f.u = cur; f.un = curn; f.uo = alt; f.uno = altn;
-            launch_fused<T>(b->order, b->bc, dyn, f, s);
+            launch_fused<T>(b->order, b->bc, dyn, b->fshort, f, s);
             std::swap(cur, alt);
             std::swap(curn, altn);
         }
@@ -1156,7 +1200,12 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     // tiles, member by member, row-major inside a member
     std::vector<TileDesc> tiles;
     int tx = 64, ty = BATCH_STAGE_TY;
-    if (engine == NLS_ENGINE_FUSED) ty = fused_ty(dtype, order);
+    if (engine == NLS_ENGINE_FUSED) {
+        long long tiles16 = 0;
+        for (int i = 0; i < count; ++i) tiles16 += (long long)((members[i].n + 63) / 64) * ((members[i].n + 15) / 16);
+        b->fshort = fused_short_for(order, tiles16);
+        ty = fused_ty(dtype, order, b->fshort);
+    }
     if (dim == 1) { tx = TILE1; ty = 1; }
     for (int i = 0; i < count; ++i) {
         const int n = members[i].n;

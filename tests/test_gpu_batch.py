@@ -19,11 +19,11 @@ def rel(a, b):
     return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
 
 
-def members_2d(sizes, dyn=True, seed=11, ref=False):
+def members_2d(sizes, dyn=True, seed=11, ref=False, spacing=None):
     out = []
     for i, n in enumerate(sizes):
         rng = np.random.default_rng(seed + i)
-        L = 12.0 + 4.0 * (i % 3)
+        L = 12.0 + 4.0 * (i % 3) if spacing is None else spacing * n
         X, Y, h = configs.grid_2d(n, L)
         r = np.sqrt(X ** 2 + Y ** 2)
         psi0 = 0.2 * np.exp(-r ** 2 / 5.0) + configs.noise(rng, r.shape, 1e-2)
@@ -52,6 +52,39 @@ def members_1d(sizes, dyn=True, seed=21, ref=False):
 
 def oracle_member(oracle, m, steps, bc, dim=2, order=5):
     return oracle.rk4(dim, m.n, order, 0 if bc == "ref" else 1, m.dt, m.dx, steps, m.P, m.coeffs, m.psi0, m.n0)
+
+
+@pytest.mark.parametrize("dtype,count", [("f32", 20), ("f64", 8)])
+@pytest.mark.parametrize("order", (3, 5))
+def test_tall_and_short_fused_tiles_agree(gpu, oracle, dtype, count, order):
+    """The fused kernel runs 16-row tiles when a launch has no more of them
+    than the device has compute units, and 32-row tiles otherwise.  A batch
+    of many 256^2 members takes the tall tiles (its 16-row tiles, 64 a
+    member, outnumber the 256 compute units of MI355X); each member alone
+    takes the short ones.  A point's update
+    is the same arithmetic in either tiling: bitwise equal.  In double
+    precision a member of the tall batch also matches the oracle."""
+    steps = 30
+    ms = members_2d([256] * count, seed=40, spacing=0.15)
+    mix = Batch(ms, dim=2, order=order, bc="periodic", dtype=dtype, engine="fused")
+    assert mix.tiles == count * 4 * 8                 # 64 x 32 tiles
+    mix.advance(steps)
+    for i in (0, count - 1):
+        alone = Batch([ms[i]], dim=2, order=order, bc="periodic", dtype=dtype, engine="fused")
+        assert alone.tiles == 4 * 16                  # 64 x 16 tiles
+        alone.advance(steps)
+        a, an = alone.member(0)
+        b, bn = mix.member(i)
+        assert np.all(np.isfinite(a))
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(an, bn)
+        alone.close()
+    if dtype == "f64":
+        psi, nres = mix.member(0)
+        opsi, onres = oracle_member(oracle, ms[0], steps, "periodic", order=order)
+        assert rel(psi, opsi) <= TOL64, rel(psi, opsi)
+        assert rel(nres, onres) <= TOL64
+    mix.close()
 
 
 @pytest.mark.parametrize("engine", ("stage", "fused"))

@@ -152,6 +152,31 @@ def test_square256_f64_first_1000_steps(gpu, oracle):
     assert rel(psi32, opsi) <= TOL32
 
 
+@pytest.mark.parametrize("n,dyn", [(640, True), (512, False)])
+def test_fused_plain_f64_both_tile_heights(gpu, oracle, n, dyn):
+    """The plain fused solve of one grid in double precision: a 640^2 grid
+    runs 32-row tiles (its 400 16-row tiles outnumber the 256 compute units
+    of MI355X), a 512^2 grid 16-row tiles (exactly 256).  Both
+    against the oracle; the small grids above cover 16-row tiles at every
+    order and boundary."""
+    steps = 12
+    oracle.set_threads(8)
+    X, Y, h = configs.grid_2d(n, 0.15 * n)
+    r2 = X ** 2 + Y ** 2
+    rng = np.random.default_rng(n)
+    psi0 = 0.3 * np.exp(-r2 / 200.0) + configs.noise(rng, (n, n), 1e-2)
+    P = 2.0 * np.exp(-r2 / 300.0)
+    n0 = 0.2 * np.exp(-r2 / 100.0) if dyn else None
+    c = configs.model_coeffs(dynamic=dyn)
+    desc = native.ProblemDesc.make(2, n, 5, 1e-3, h, c, bc="periodic", dtype="f64", engine="fused")
+    psi, nres = native.solve_host(desc, steps, P, psi0, n0)
+    opsi, onres = oracle.rk4(2, n, 5, 1, 1e-3, h, steps, P, c, psi0, n0)
+    assert np.all(np.isfinite(psi))
+    assert rel(psi, opsi) <= TOL64, rel(psi, opsi)
+    if dyn:
+        assert rel(nres, onres) <= TOL64
+
+
 @pytest.mark.parametrize("dim", (1, 2))
 def test_dropin_chemical_potential(gpu, oracle, dim):
     n = 64 if dim == 1 else 32

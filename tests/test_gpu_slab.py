@@ -53,6 +53,7 @@ def whole_grid(desc, Pt, state, steps):
     (96, 5, "f32", True, 1),
     (48, 5, "f64", True, [24, 24]),
     (48, 3, "f32", False, [8, 8, 16, 16]),
+    (640, 5, "f64", True, [320, 320]),        # whole grid in 32-row tiles, slabs in 16-row tiles
 ])
 def test_slabs_equal_whole_grid(gpu, n, order, dtype, dyn, rows):
     import torch
