@@ -158,33 +158,44 @@ template <typename T> struct FusedArgs {
 // double-buffered in LDS (stage s reads one buffer and writes the next
 // stage's input into the other: one barrier per stage and no registers
 // holding the next input).  WAVES: waves per SIMD the register budget is
-// sized for.  SHORT: 16-row tiles at every order, for launches with no more
-// tiles than the device has compute units (one lone grid, a few small
-// members): the step time is then one workgroup's critical path, which grows
-// with the region points per thread (2.5 instead of 3.75 at orders 3 and 5).
-// Launches of more tiles use the taller tiles, whose halo costs less per
-// output point.
-template <typename T, int ORDER, bool SHORT = false> struct FusedTile;
-template <int ORDER, bool SHORT> struct FusedTile<float, ORDER, SHORT> {
-    static constexpr int TY = ORDER == 7 || SHORT ? 16 : 32;
+// sized for.
+//
+// FORM, chosen per launch by fused_form():
+//  - FORM_TALL: 64 x 32 tiles of 1024 threads (64 x 16 at order 7), for
+//    launches of several rounds, where the tall tile's smaller halo per
+//    output point pays, or whose tall tiles fit one per compute unit.
+//  - FORM_SHORT: 64 x 16 tiles of 1024 threads, for launches with no more
+//    such tiles than the device has compute units (a lone grid, a few small
+//    members): the step is then one workgroup's critical path, which grows
+//    with the region points per thread (2.5 instead of 3.75).
+//  - FORM_HALF (double precision, periodic grids and row slabs): 64 x 16
+//    tiles of 512 threads with a single stage buffer, 48 KB of LDS, so a CU
+//    holds two workgroups (the 1024-thread 64 x 32 tile needs 137 KB and a
+//    CU holds one) and one workgroup's loads overlap the other's arithmetic,
+//    which outweighs the larger halo when the launch runs more than two
+//    rounds.
+//    With reflecting boundaries this form would spill registers.
+enum { FORM_TALL = 0, FORM_SHORT = 1, FORM_HALF = 2 };
+template <typename T, int ORDER, int FORM> struct FusedTile;
+template <int ORDER, int FORM> struct FusedTile<float, ORDER, FORM> {
+    static_assert(FORM != FORM_HALF, "single precision has no half-size form");
+    static constexpr int TY = ORDER == 7 || FORM == FORM_SHORT ? 16 : 32;
     static constexpr int NT = 1024;
     static constexpr bool DB = true;
     static constexpr int WAVES = 8;
 };
-template <int ORDER, bool SHORT> struct FusedTile<double, ORDER, SHORT> {
-    static constexpr int TY = ORDER == 7 || SHORT ? 16 : 32;
-    static constexpr int NT = 1024;
-    static constexpr bool DB = true;
+template <int ORDER, int FORM> struct FusedTile<double, ORDER, FORM> {
+    static constexpr bool HALF = FORM == FORM_HALF;
+    static constexpr int TY = ORDER == 7 || FORM != FORM_TALL ? 16 : 32;
+    static constexpr int NT = HALF ? 512 : 1024;
+    static constexpr bool DB = !HALF;
     static constexpr int WAVES = 4;
 };
 // LDS row stride of the region (W + pad), chosen so that the lanes of the
 // ring order's side bands (2R points per row over consecutive rows) fall in
 // distinct banks.
 constexpr int fused_lds_stride(int W, int R) { return W + (R == 1 ? 2 : R == 2 ? 4 : 14); }
-static int fused_ty(int dtype, int order, bool shrt) {
-    if (shrt || order == 7) return 16;
-    return dtype == NLS_F64 ? FusedTile<double, 5>::TY : FusedTile<float, 5>::TY;
-}
+static int fused_ty(int order, int form) { return order == 7 || form != FORM_TALL ? 16 : 32; }
 
 // Region points in ring order, built at compile time: entry k holds the
 // point's LDS index y WS + x in bits 0..15, x in bits 16..23, y in 24..31.
@@ -207,11 +218,11 @@ static_assert(RingMap<80, 48, 2, 84>().v[2048] == (6 * 84 + 6 | 6 << 16 | 6 << 2
 static_assert(RingMap<80, 32, 2, 84>().v[1024] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the 64 x 16 tile");
 template <int W, int HH, int R, int WS> __device__ const RingMap<W, HH, R, WS> ring_map{};
 
-template <typename T, int ORDER, int BC, bool DYN, bool SHORT>
-__global__ void __launch_bounds__((FusedTile<T, ORDER, SHORT>::NT), (FusedTile<T, ORDER, SHORT>::WAVES))
+template <typename T, int ORDER, int BC, bool DYN, int FORM>
+__global__ void __launch_bounds__((FusedTile<T, ORDER, FORM>::NT), (FusedTile<T, ORDER, FORM>::WAVES))
 k_fused_step_2d(const FusedArgs<T> f) {
     using C = cplx_t<T>;
-    using FT = FusedTile<T, ORDER, SHORT>;
+    using FT = FusedTile<T, ORDER, FORM>;
     constexpr int R = (ORDER - 1) / 2, H = 4 * R;
     constexpr int TX = 64, TY = FT::TY, NT = FT::NT;
     constexpr bool DB = FT::DB;
@@ -665,20 +676,33 @@ static void launch_batch_stage(int dim, int order, int bc, bool dyn, const Batch
 #undef NLS_CASE
 }
 
-// shrt: the 16-row tiles of orders 3 and 5 (order 7 has only 16-row tiles).
-template <typename T>
-static void launch_fused(int order, int bc, bool dyn, bool shrt, const FusedArgs<T> &f, hipStream_t s) {
+// One fused step launch in the given form; forms a configuration does not
+// build (order 7 has one form; FORM_HALF is double precision without
+// reflecting boundaries) fall back to FORM_TALL, which fused_form never asks
+// otherwise.
+template <typename T, int O, int B, bool D>
+static void launch_fused_form(int form, const FusedArgs<T> &f, hipStream_t s) {
     const dim3 grid(8 * f.chunk);
-    shrt = shrt && order != 7;
-#define NLS_CASE(O, B, D)                                                                                       \
-    if (order == O && bc == B && dyn == D) {                                                                   \
-        if (shrt)                                                                                              \
-            hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, O != 7>), grid, dim3(FusedTile<T, O, true>::NT), 0, \
-                               s, f);                                                                          \
-        else                                                                                                   \
-            hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, false>), grid, dim3(FusedTile<T, O>::NT), 0, s, f); \
-        return;                                                                                                \
+    if constexpr (O != 7) {
+        if (form == FORM_SHORT) {
+            hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_SHORT>), grid, dim3(FusedTile<T, O, FORM_SHORT>::NT),
+                               0, s, f);
+            return;
+        }
+        if constexpr (sizeof(T) == 8 && B != BC_REF)
+            if (form == FORM_HALF) {
+                hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_HALF>), grid,
+                                   dim3(FusedTile<T, O, FORM_HALF>::NT), 0, s, f);
+                return;
+            }
     }
+    hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_TALL>), grid, dim3(FusedTile<T, O, FORM_TALL>::NT), 0, s, f);
+}
+
+template <typename T>
+static void launch_fused(int order, int bc, bool dyn, int form, const FusedArgs<T> &f, hipStream_t s) {
+#define NLS_CASE(O, B, D) \
+    if (order == O && bc == B && dyn == D) return launch_fused_form<T, O, B, D>(form, f, s);
     NLS_CASE(3, 0, false) NLS_CASE(3, 0, true) NLS_CASE(3, 1, false) NLS_CASE(3, 1, true)
     NLS_CASE(5, 0, false) NLS_CASE(5, 0, true) NLS_CASE(5, 1, false) NLS_CASE(5, 1, true)
     NLS_CASE(7, 0, false) NLS_CASE(7, 0, true) NLS_CASE(7, 1, false) NLS_CASE(7, 1, true)
@@ -699,19 +723,31 @@ static int device_cus() {
     return v;
 }
 
-// Whether a fused launch over `tiles16` 16-row tiles runs them short: when
-// each gets a compute unit of its own, the step takes one short workgroup's
-// time (measured on MI355X, order 5: a lone 256^2 grid 7.1 -> 5.2 us per step
-// in single precision, 8.8 -> 6.2 us in double; 512^2, 256 short tiles,
-// 7.3 -> 5.7 us and 9.4 -> 7.1 us).  Where compute units would hold two
-// short tiles the taller tiles are as fast or faster (a 2048 x 256 row slab:
-// 9.7 us short, 9.5 us tall), and launches of several rounds are faster tall
-// (a batch of 512^2-class grids: 94.6 us tall, 112.4 us short).
-static bool fused_short_for(int order, long long tiles16) {
-    if (order == 7) return false;
+// The fused kernel's form for a launch over grids (or row slabs) of the
+// given sizes, from the count of its 64 x 16 tiles (tiles16) and 64 x 32
+// tiles (tiles32) against the compute units.  Measured on MI355X, order 5:
+//  - tiles16 <= CUs, FORM_SHORT: a lone 256^2 grid 7.1 -> 5.2 us per step in
+//    single precision and 8.8 -> 6.2 us in double; 512^2 (256 tiles)
+//    7.3 -> 5.7 and 9.4 -> 7.1 us.
+//  - otherwise FORM_TALL: where CUs would hold two short tiles the tall
+//    ones are as fast or faster (a 2048 x 256 row slab: 9.7 us short, 9.5 us
+//    tall), and launches of several rounds are faster tall (a batch of
+//    512^2-class grids: 94.6 us tall, 112.4 us short);
+//  - except in double precision without reflecting boundaries when the tall
+//    tiles take more than two rounds of the CUs, FORM_HALF (alternating runs
+//    on one device): 2048^2 (8 rounds) 89 -> 82 us, the sweep512 batch
+//    196 -> 190 us; at two rounds or fewer the tall tiles are as fast or
+//    faster (1024^2: 41 us tall, 43 us half; a 2048 x 256 slab, one round:
+//    13.1 us tall, 14.9 us half).
+static int fused_form(int dtype, int order, int bc, long long tiles16, long long tiles32) {
+    if (order == 7) return FORM_TALL;
     const int cus = device_cus();
-    return cus > 0 && tiles16 <= cus;
+    if (cus <= 0) return FORM_TALL;
+    if (tiles16 <= cus) return FORM_SHORT;
+    if (dtype == NLS_F64 && bc != BC_REF && tiles32 > 2LL * cus) return FORM_HALF;
+    return FORM_TALL;
 }
+static long long tiles_of(int n, int rows, int ty) { return (long long)((n + 63) / 64) * ((rows + ty - 1) / ty); }
 
 // The double-precision 2-D order-7 variant would spill registers to scratch;
 // it is not built and such batches use the fused or stage engine.
@@ -777,8 +813,8 @@ static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T>
     using C = cplx_t<T>;
     const size_t N = (size_t)p->n * p->n;
     const bool dyn = p->coeffs[10] != 0.0;
-    const bool shrt = fused_short_for(p->order, (long long)((p->n + 63) / 64) * ((p->n + 15) / 16));
-    const int ty = fused_ty(p->dtype, p->order, shrt);
+    const int form = fused_form(p->dtype, p->order, p->bc, tiles_of(p->n, p->n, 16), tiles_of(p->n, p->n, 32));
+    const int ty = fused_ty(p->order, form);
     FusedArgs<T> f;
     memset(&f, 0, sizeof f);
     f.grid_ntx = (p->n + 63) / 64;
@@ -791,7 +827,7 @@ static int fused_grid_run(const nls_problem *p, int iters, const T *P, cplx_t<T>
     for (int it = 0; it < iters; ++it) {
         f.it = it;
         f.u = cur; f.un = curn; f.uo = alt; f.uno = altn;
-        launch_fused<T>(p->order, p->bc, dyn, shrt, f, s);
+        launch_fused<T>(p->order, p->bc, dyn, form, f, s);
         std::swap(cur, alt);
         std::swap(curn, altn);
     }
@@ -814,8 +850,8 @@ template <typename T>
 static int fused_slab_run(const nls_problem *p, int rows, const T *P, const cplx_t<T> *psi_in, const T *nres_in,
                           cplx_t<T> *psi_out, T *nres_out, hipStream_t s) {
     const int H = 2 * (p->order - 1);
-    const bool shrt = fused_short_for(p->order, (long long)((p->n + 63) / 64) * ((rows + 15) / 16));
-    const int ty = fused_ty(p->dtype, p->order, shrt);
+    const int form = fused_form(p->dtype, p->order, BC_SLAB, tiles_of(p->n, rows, 16), tiles_of(p->n, rows, 32));
+    const int ty = fused_ty(p->order, form);
     FusedArgs<T> f;
     memset(&f, 0, sizeof f);
     f.grid_ntx = (p->n + 63) / 64;
@@ -825,7 +861,7 @@ static int fused_slab_run(const nls_problem *p, int rows, const T *P, const cplx
     f.slab_end = H + rows;
     f.P = P;
     f.u = psi_in; f.un = nres_in; f.uo = psi_out; f.uno = nres_out;
-    launch_fused<T>(p->order, BC_SLAB, p->coeffs[10] != 0.0, shrt, f, s);
+    launch_fused<T>(p->order, BC_SLAB, p->coeffs[10] != 0.0, form, f, s);
     return hip_check(hipGetLastError(), "slab step launch");
 }
 
@@ -914,7 +950,7 @@ struct nls_batch {
     size_t mem_stride;
     std::vector<double> h_dx;           // grid spacing of each member
     int min_steps = 0x7fffffff;         // smallest member step budget
-    bool fshort = false;                // fused engine: 16-row tiles
+    int fform = FORM_TALL;              // fused engine: tile form (fused_form)
     int flags = 0;                      // NLS_FLAG_EAGER: no step-block graphs
     // the captured step block and the buffers it was captured on
     hipGraphExec_t graph = nullptr;
@@ -986,7 +1022,7 @@ static int batch_steps(nls_batch *b, int it0, int iters, const T *P, cplx_t<T> *
         for (int it = 0; it < iters; ++it) {
             f.it = it0 + it;
             f.u = cur; f.un = curn; f.uo = alt; f.uno = altn;
-            launch_fused<T>(b->order, b->bc, dyn, b->fshort, f, s);
+            launch_fused<T>(b->order, b->bc, dyn, b->fform, f, s);
             std::swap(cur, alt);
             std::swap(curn, altn);
         }
@@ -1201,10 +1237,13 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     std::vector<TileDesc> tiles;
     int tx = 64, ty = BATCH_STAGE_TY;
     if (engine == NLS_ENGINE_FUSED) {
-        long long tiles16 = 0;
-        for (int i = 0; i < count; ++i) tiles16 += (long long)((members[i].n + 63) / 64) * ((members[i].n + 15) / 16);
-        b->fshort = fused_short_for(order, tiles16);
-        ty = fused_ty(dtype, order, b->fshort);
+        long long tiles16 = 0, tiles32 = 0;
+        for (int i = 0; i < count; ++i) {
+            tiles16 += tiles_of(members[i].n, members[i].n, 16);
+            tiles32 += tiles_of(members[i].n, members[i].n, 32);
+        }
+        b->fform = fused_form(dtype, order, bc, tiles16, tiles32);
+        ty = fused_ty(order, b->fform);
     }
     if (dim == 1) { tx = TILE1; ty = 1; }
     for (int i = 0; i < count; ++i) {

@@ -54,23 +54,25 @@ def oracle_member(oracle, m, steps, bc, dim=2, order=5):
     return oracle.rk4(dim, m.n, order, 0 if bc == "ref" else 1, m.dt, m.dx, steps, m.P, m.coeffs, m.psi0, m.n0)
 
 
-@pytest.mark.parametrize("dtype,count", [("f32", 20), ("f64", 8)])
+@pytest.mark.parametrize("dtype,bc,count,rows", [("f32", "periodic", 20, 32), ("f64", "periodic", 20, 16),
+                                                 ("f64", "periodic", 12, 32), ("f64", "ref", 20, 32)])
 @pytest.mark.parametrize("order", (3, 5))
-def test_tall_and_short_fused_tiles_agree(gpu, oracle, dtype, count, order):
-    """The fused kernel runs 16-row tiles when a launch has no more of them
-    than the device has compute units, and 32-row tiles otherwise.  A batch
-    of many 256^2 members takes the tall tiles (its 16-row tiles, 64 a
-    member, outnumber the 256 compute units of MI355X); each member alone
-    takes the short ones.  A point's update
-    is the same arithmetic in either tiling: bitwise equal.  In double
-    precision a member of the tall batch also matches the oracle."""
+def test_fused_tile_forms_agree(gpu, oracle, dtype, bc, count, rows, order):
+    """The fused kernel's tile form is chosen per launch against the 256
+    compute units of MI355X: 1024-thread 64 x 16 tiles when they fit one a
+    CU (each member alone here: 64 of them), else 1024-thread 64 x 32 tiles,
+    except in double precision on periodic grids when those take more than
+    two rounds of the CUs (20 members: 640), which runs 512-thread 64 x 16
+    tiles, two workgroups a CU.  A point's update is the same arithmetic in every form:
+    each member of the batch equals the member alone bit for bit.  In double
+    precision a member of the batch also matches the oracle."""
     steps = 30
     ms = members_2d([256] * count, seed=40, spacing=0.15)
-    mix = Batch(ms, dim=2, order=order, bc="periodic", dtype=dtype, engine="fused")
-    assert mix.tiles == count * 4 * 8                 # 64 x 32 tiles
+    mix = Batch(ms, dim=2, order=order, bc=bc, dtype=dtype, engine="fused")
+    assert mix.tiles == count * 4 * (256 // rows)
     mix.advance(steps)
     for i in (0, count - 1):
-        alone = Batch([ms[i]], dim=2, order=order, bc="periodic", dtype=dtype, engine="fused")
+        alone = Batch([ms[i]], dim=2, order=order, bc=bc, dtype=dtype, engine="fused")
         assert alone.tiles == 4 * 16                  # 64 x 16 tiles
         alone.advance(steps)
         a, an = alone.member(0)
@@ -81,7 +83,7 @@ def test_tall_and_short_fused_tiles_agree(gpu, oracle, dtype, count, order):
         alone.close()
     if dtype == "f64":
         psi, nres = mix.member(0)
-        opsi, onres = oracle_member(oracle, ms[0], steps, "periodic", order=order)
+        opsi, onres = oracle_member(oracle, ms[0], steps, bc, order=order)
         assert rel(psi, opsi) <= TOL64, rel(psi, opsi)
         assert rel(nres, onres) <= TOL64
     mix.close()

@@ -152,13 +152,14 @@ def test_square256_f64_first_1000_steps(gpu, oracle):
     assert rel(psi32, opsi) <= TOL32
 
 
-@pytest.mark.parametrize("n,dyn", [(640, True), (512, False)])
-def test_fused_plain_f64_both_tile_heights(gpu, oracle, n, dyn):
-    """The plain fused solve of one grid in double precision: a 640^2 grid
-    runs 32-row tiles (its 400 16-row tiles outnumber the 256 compute units
-    of MI355X), a 512^2 grid 16-row tiles (exactly 256).  Both
-    against the oracle; the small grids above cover 16-row tiles at every
-    order and boundary."""
+@pytest.mark.parametrize("n,dyn", [(1152, True), (640, True), (512, False)])
+def test_fused_plain_f64_tile_forms(gpu, oracle, n, dyn):
+    """The plain fused solve of one grid in double precision, in each tile
+    form (on the 256 compute units of MI355X): 1152^2 in 512-thread 64 x 16
+    tiles, two workgroups a CU (its 648 64 x 32 tiles take three rounds);
+    640^2 in 1024-thread 64 x 32 tiles (200 of them); 512^2 in 1024-thread
+    64 x 16 tiles (256).  Each against the oracle; the small grids above
+    cover the 64 x 16 form at every order and boundary."""
     steps = 12
     oracle.set_threads(8)
     X, Y, h = configs.grid_2d(n, 0.15 * n)

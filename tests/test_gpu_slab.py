@@ -53,7 +53,9 @@ def whole_grid(desc, Pt, state, steps):
     (96, 5, "f32", True, 1),
     (48, 5, "f64", True, [24, 24]),
     (48, 3, "f32", False, [8, 8, 16, 16]),
-    (640, 5, "f64", True, [320, 320]),        # whole grid in 32-row tiles, slabs in 16-row tiles
+    (640, 5, "f64", True, [320, 320]),        # whole grid in 64 x 32 tiles, slabs in 64 x 16
+    (640, 5, "f32", True, [320, 320]),        # the same
+    (1152, 5, "f64", True, [576, 576]),       # whole grid in 512-thread 64 x 16 tiles, slabs in 64 x 32
 ])
 def test_slabs_equal_whole_grid(gpu, n, order, dtype, dyn, rows):
     import torch
