@@ -273,7 +273,7 @@ def main():
     ap.add_argument("--serve-rates", default="20,50,100,200,500",
                     help="Poisson arrival rates (requests/s, comma-separated), one request-loop run each")
     ap.add_argument("--serve-window", type=float, default=2.0, help="arrival window (s)")
-    ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "stage", "resident", "auto"),
+    ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "hybrid", "stage", "resident", "auto"),
                     help="batch engine of the headline workload (tuned: measured on this device)")
     ap.add_argument("--grid2048-only", action="store_true",
                     help="only measure the 2048 x 2048 single grid (plain path) and print its JSON")
@@ -350,12 +350,18 @@ def main():
     launch_s = gpu_s / K             # one fused-kernel launch per step; events on its stream
     log("batched FP32: %.3fs wall, %.1f us/step, %.2f G grid-point-steps/s" % (wall, wall / K * 1e6, value / 1e9))
 
-    if engine == "fused":
-        bytes_per_launch = points * FUSED_BYTES_F32_DYN
+    if engine in ("fused", "hybrid"):
+        # hybrid: the 64^2 members run on the resident engine on a side stream
+        # (state on chip, HBM at the ends of a call); the fused kernel, the
+        # only one on this stream, steps the others
+        fused_points = points if engine == "fused" else sum(m.points for m in members if m.n * m.n > 4096)
+        bytes_per_launch = fused_points * FUSED_BYTES_F32_DYN
         achieved = bytes_per_launch / launch_s
+        traffic = read_traffic("fused_step_f32_sweep512_traffic.txt" if engine == "fused" else
+                               "fused_step_f32_sweep512_hybrid_traffic.txt")
         roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                    "frac": achieved / HBM_PEAK, "traffic": read_traffic("fused_step_f32_sweep512_traffic.txt"),
-                    "kernel": "k_fused_step_2d<float,5,periodic,dynamic>",
+                    "frac": achieved / HBM_PEAK, "traffic": traffic,
+                    "kernel": "k_fused_step_2d<float,5,periodic,dynamic>", "kernel_points": fused_points,
                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6}
     else:
         # per-stage engine: four launches per step, 15 S + 4 r bytes per point-step
@@ -371,7 +377,10 @@ def main():
            "config": {"workload": "sweep512: 512 2-D instances, sides 64/128/256 (p=0.5/0.3/0.2), periodic, "
                                   "order 5, dynamic reservoir, dt=5e-4, %d RK4 steps, complex64 state" % K,
                       "instances": len(members), "grid_points": points, "engine": engine,
-                      "step_blocks": "64-step hipGraph replays" if engine != "resident" else "one launch per call",
+                      "step_blocks": {"resident": "one launch per call",
+                                      "hybrid": "fused part: 64-step hipGraph replays; resident part (64^2 members, "
+                                                "own stream): one launch per call"}.get(engine,
+                                                                                       "64-step hipGraph replays"),
                       "parallelism": "single GPU"},
            "roofline": roofline}
     if tuning is not None:

@@ -53,7 +53,8 @@ extern "C" {
 /* RK4 engines (nls_problem.engine, nls_batch_create) */
 #define NLS_ENGINE_AUTO  -1   /* RESIDENT if the instance (every member of a
                                  batch) fits, else FUSED for 2-D and STAGE
-                                 for 1-D                                     */
+                                 for 1-D; a batch of which only some members
+                                 fit: HYBRID                                 */
 #define NLS_ENGINE_STAGE  0   /* one launch per RK4 stage                    */
 #define NLS_ENGINE_FUSED  1   /* one launch per RK4 step, halo recomputed on
                                  chip (2-D only)                             */
@@ -61,6 +62,13 @@ extern "C" {
                                  chip and runs a block of steps per launch
                                  (1-D n <= 4096, 2-D n <= 64; not FP64 2-D
                                  order 7)                                    */
+#define NLS_ENGINE_HYBRID 3   /* batches only: the members that fit on the
+                                 RESIDENT engine run there, on a stream of
+                                 the batch forked from and joined to the
+                                 caller's, beside the others on FUSED (2-D)
+                                 or STAGE (1-D); no counterpart in the
+                                 reference, which solves one instance per
+                                 process                                     */
 
 const char *nls_last_error(void);
 

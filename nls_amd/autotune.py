@@ -169,6 +169,9 @@ class Tuner(object):
         key = self._key("batch", dim=dim, order=order, bc=bc, dtype=dtype, dyn=dyn, mix=mix)
         cands = [e for e in (ENGINES_2D if dim == 2 else ENGINES_1D)
                  if e != "resident" or _batch_resident_fits(members, dim, order, dtype)]
+        fit = sum(_batch_resident_fits([m], dim, order, dtype) for m in members)
+        if 0 < fit < len(members):
+            cands.append("hybrid")           # the members that fit resident, the rest on the fused/per-stage engine
         return self._choose(key, cands, lambda e: self._time_batch(members, dim, order, bc, dtype, e))
 
     def _time_batch(self, members, dim, order, bc, dtype, engine):

@@ -956,7 +956,17 @@ struct nls_batch {
     hipGraphExec_t graph = nullptr;
     const void *g_P = nullptr, *g_psi = nullptr, *g_nres = nullptr, *g_ws = nullptr;
     hipStream_t capture = nullptr;
+    // NLS_ENGINE_HYBRID: the members the resident engine holds (part 0) and
+    // the others (part 1: fused in 2-D, per-stage in 1-D), each a batch of
+    // its own over the same packed arrays (member offsets unchanged); ids[k]
+    // lists part k's members by index in this batch.  Part 0 runs on `side`,
+    // forked from the caller's stream and joined back to it.
+    nls_batch *part[2] = {nullptr, nullptr};
+    std::vector<int> ids[2];
+    hipStream_t side = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
 };
+static bool is_hybrid(const nls_batch *b) { return b->part[1] != nullptr; }
 
 // Step-block graphs of a batch (fused and per-stage engines): a block of
 // BATCH_GRAPH_BLOCK steps (even: the state ping-pong ends in the caller's
@@ -1188,12 +1198,14 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     if (dtype != NLS_F32 && dtype != NLS_F64) return fail(NLS_EINVAL, "bad dtype");
     const int res_max = dim == 1 ? 4096 : 64;
     const bool res_built = !(dim == 2 && dtype == NLS_F64 && order == 7);
-    if (engine == NLS_ENGINE_AUTO) {
-        bool all_fit = res_built;
-        for (int i = 0; i < count; ++i) all_fit = all_fit && members[i].n <= res_max;
-        engine = all_fit ? NLS_ENGINE_RESIDENT : (dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE);
-    }
-    if (engine != NLS_ENGINE_STAGE && engine != NLS_ENGINE_FUSED && engine != NLS_ENGINE_RESIDENT)
+    int fit = 0;
+    for (int i = 0; i < count; ++i) fit += res_built && members[i].n <= res_max;
+    const int rest_engine = dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
+    if (engine == NLS_ENGINE_AUTO) engine = fit == count ? NLS_ENGINE_RESIDENT : fit > 0 ? NLS_ENGINE_HYBRID : rest_engine;
+    if (engine == NLS_ENGINE_HYBRID && (fit == 0 || fit == count))      // one part only
+        engine = fit == count ? NLS_ENGINE_RESIDENT : rest_engine;
+    if (engine != NLS_ENGINE_STAGE && engine != NLS_ENGINE_FUSED && engine != NLS_ENGINE_RESIDENT &&
+        engine != NLS_ENGINE_HYBRID)
         return fail(NLS_EINVAL, "bad engine");
     if (engine == NLS_ENGINE_RESIDENT && !res_built)
         return fail(NLS_EUNSUPPORTED, "the resident engine has no 2-D order-7 double-precision variant");
@@ -1225,6 +1237,28 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
         std::sort(iv.begin(), iv.end());
         for (size_t i = 1; i < iv.size(); ++i)
             if (iv[i].first < iv[i - 1].second) return fail(NLS_EINVAL, "members overlap in the packed arrays");
+    }
+    if (engine == NLS_ENGINE_HYBRID) {
+        nls_batch *b = new nls_batch();
+        b->dim = dim; b->order = order; b->bc = bc; b->dtype = dtype; b->engine = engine; b->dyn = dyn;
+        b->count = count; b->points = points;
+        std::vector<nls_member> ms[2];
+        for (int i = 0; i < count; ++i) {
+            b->h_dx.push_back(members[i].dx);
+            const int k = res_built && members[i].n <= res_max ? 0 : 1;
+            b->ids[k].push_back(i);
+            ms[k].push_back(members[i]);
+        }
+        int rc = nls_batch_create(dim, order, bc, dtype, NLS_ENGINE_RESIDENT, ms[0].data(), (int)ms[0].size(),
+                                  &b->part[0]);
+        if (!rc) rc = nls_batch_create(dim, order, bc, dtype, rest_engine, ms[1].data(), (int)ms[1].size(), &b->part[1]);
+        if (!rc) rc = hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hybrid stream");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "hybrid event");
+        if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming), "hybrid event");
+        if (rc) { nls_batch_destroy(b); return rc; }
+        b->ntiles = b->part[1]->ntiles;
+        *out = b;
+        return NLS_OK;
     }
     nls_batch *b = new nls_batch();
     b->dim = dim; b->order = order; b->bc = bc; b->dtype = dtype; b->engine = engine; b->dyn = dyn;
@@ -1284,6 +1318,10 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
 
 int nls_batch_destroy(nls_batch *b) {
     if (!b) return NLS_OK;
+    for (nls_batch *q : b->part) nls_batch_destroy(q);
+    if (b->side) (void)hipStreamDestroy(b->side);
+    if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
+    if (b->join_ev) (void)hipEventDestroy(b->join_ev);
     (void)drop_graph(b);
     if (b->capture) (void)hipStreamDestroy(b->capture);
     if (b->d_mem) (void)hipFree(b->d_mem);
@@ -1299,6 +1337,7 @@ int nls_batch_tiles(const nls_batch *b) { return b ? b->ntiles : 0; }
 
 size_t nls_batch_workspace_bytes(const nls_batch *b) {
     if (!b) return 0;
+    if (is_hybrid(b)) return nls_batch_workspace_bytes(b->part[1]);     // the resident part needs none
     const size_t rs = b->dtype == NLS_F64 ? 8 : 4;
     const size_t cb = align256b(b->points * 2 * rs), rb = align256b(b->points * rs);
     if (b->engine == NLS_ENGINE_RESIDENT) return 0;
@@ -1310,6 +1349,16 @@ int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream) {
     if (!b || !steps) return fail(NLS_EINVAL, "NULL argument");
     for (int i = 0; i < b->count; ++i)
         if (steps[i] < 0) return fail(NLS_EINVAL, "member %d: negative step budget", i);
+    if (is_hybrid(b)) {
+        // the caller's stream has joined every launch of the side stream
+        for (int k = 0; k < 2; ++k) {
+            std::vector<int32_t> st;
+            for (int i : b->ids[k]) st.push_back(steps[i]);
+            const int rc = nls_batch_set_steps(b->part[k], st.data(), stream);
+            if (rc) return rc;
+        }
+        return NLS_OK;
+    }
     b->min_steps = 0x7fffffff;
     for (int i = 0; i < b->count; ++i) {
         if (b->dtype == NLS_F64) set_member_steps<double>(b, i, steps[i]);
@@ -1325,6 +1374,16 @@ int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream) {
 
 int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, double *out, void *stream) {
     if (!b || !psi || !out) return fail(NLS_EINVAL, "NULL argument");
+    if (is_hybrid(b)) {
+        for (int k = 0; k < 2; ++k) {
+            std::vector<double> o(b->ids[k].size() * 4);
+            const int rc = nls_batch_observables(b->part[k], psi, nres, o.data(), stream);
+            if (rc) return rc;
+            for (size_t q = 0; q < b->ids[k].size(); ++q)
+                for (int c = 0; c < 4; ++c) out[4 * b->ids[k][q] + c] = o[4 * q + c];
+        }
+        return NLS_OK;
+    }
     hipStream_t s = (hipStream_t)stream;
     const int radial = (b->dim == 1 && b->bc == NLS_BC_REF) ? 1 : 0;
     void *d_out = nullptr;
@@ -1362,6 +1421,7 @@ int nls_batch_prepare(nls_batch *b, int iters, const void *pumping, void *psi, v
     const size_t need = nls_batch_workspace_bytes(b);
     if (workspace_bytes < need || (need && !workspace))
         return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
+    if (is_hybrid(b)) return nls_batch_prepare(b->part[1], iters, pumping, psi, nres, workspace, workspace_bytes);
     if (!batch_graphs_apply(b, iters)) return NLS_OK;
     if (b->dtype == NLS_F64)
         return batch_graph<double>(b, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)workspace);
@@ -1372,6 +1432,8 @@ int nls_batch_set_flags(nls_batch *b, int flags) {
     if (!b) return fail(NLS_EINVAL, "NULL batch");
     if (flags & ~NLS_FLAG_EAGER) return fail(NLS_EINVAL, "unknown flags 0x%x", flags);
     b->flags = flags;
+    for (nls_batch *q : b->part)
+        if (q) q->flags = flags;
     return NLS_OK;
 }
 
@@ -1385,6 +1447,17 @@ int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, v
         return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
     if (iters == 0) return NLS_OK;
     hipStream_t s = (hipStream_t)stream;
+    if (is_hybrid(b)) {
+        // resident part on the side stream, the rest on the caller's, joined
+        int rc = hip_check(hipEventRecord(b->fork_ev, s), "hybrid fork");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(b->side, b->fork_ev, 0), "hybrid fork");
+        if (!rc) rc = nls_batch_advance(b->part[0], iters, pumping, psi, nres, workspace, workspace_bytes, b->side);
+        const int rc1 = rc ? rc : nls_batch_advance(b->part[1], iters, pumping, psi, nres, workspace, workspace_bytes, s);
+        // join even after a failed launch, so the caller's stream never runs ahead of the side stream
+        int rj = hip_check(hipEventRecord(b->join_ev, b->side), "hybrid join");
+        if (!rj) rj = hip_check(hipStreamWaitEvent(s, b->join_ev, 0), "hybrid join");
+        return rc1 ? rc1 : rj;
+    }
     if (b->dtype == NLS_F64)
         return batch_advance<double>(b, iters, (const double *)pumping, (double2 *)psi, (double *)nres,
                                      (char *)workspace, s);

@@ -43,6 +43,8 @@ int check_problem(const nls_problem *p) {
     if (p->scheme == NLS_SCHEME_SSF && p->bc != NLS_BC_PERIODIC)
         return fail(NLS_EUNSUPPORTED, "the split-step Fourier scheme needs periodic boundaries");
     if (p->scheme == NLS_SCHEME_SSF) return ssf_check(p);
+    if (p->engine == NLS_ENGINE_HYBRID)
+        return fail(NLS_EINVAL, "NLS_ENGINE_HYBRID splits a batch between engines; a single instance takes one engine");
     if (p->engine < NLS_ENGINE_AUTO || p->engine > NLS_ENGINE_RESIDENT)
         return fail(NLS_EINVAL, "engine must be one of NLS_ENGINE_* (got %d)", p->engine);
     if (p->engine == NLS_ENGINE_FUSED && p->dim != 2)

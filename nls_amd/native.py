@@ -38,7 +38,7 @@ EXPORTS = (
     "nls_revervoir_2d", "nls_hamiltonian", "nls_hamiltonian_2d", "nls_runge_kutta", "nls_runge_kutta_2d",
     "nls_batch_set_flags", "nls_batch_observables", "nls_batch_prepare",
 )
-ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2}
+ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2, "hybrid": 3}
 FLAG_EAGER = 1
 
 

@@ -51,9 +51,23 @@ def test_stage_tile_and_batch_choices():
     T = autotune.Tuner(path=None, timer=lambda cand, _s: {4: 4.0, 8: 3.0, 16: 1.0, 32: 2.0}[cand[1]], device_name="dev")
     assert T.stage_tile("f32") == 16
     ms = configs.sweep512(count=4, steps=10)
-    T = autotune.Tuner(path=None, timer=fake_timer({"resident": 9, "fused": 1, "stage": 2}, calls), device_name="dev")
+    T = autotune.Tuner(path=None, timer=fake_timer({"resident": 9, "fused": 1, "stage": 2, "hybrid": 3}, calls),
+                       device_name="dev")
     assert T.batch_engine(ms) == "fused"
     assert all(c[1] != "resident" for c in calls) or all(m.n == 64 for m in ms)
+    # a mix of members that fit the resident engine and members that do not
+    # adds the hybrid engine to the candidates; all fit: resident, no hybrid
+    mixed = [m for m in configs.sweep512(count=64, steps=10)]
+    assert any(m.n <= 64 for m in mixed) and any(m.n > 64 for m in mixed)
+    calls.clear()
+    T = autotune.Tuner(path=None, timer=fake_timer({"resident": 9, "fused": 2, "stage": 3, "hybrid": 1}, calls),
+                       device_name="dev")
+    assert T.batch_engine(mixed) == "hybrid"
+    assert sorted({c[1] for c in calls if c[0] == "batch"}) == ["fused", "hybrid", "stage"]
+    small = [m for m in mixed if m.n <= 64]
+    calls.clear()
+    assert T.batch_engine(small) == "fused"               # fused 2 < stage 3 < resident 9
+    assert sorted({c[1] for c in calls if c[0] == "batch"}) == ["fused", "resident", "stage"]
 
 
 @pytest.mark.gpu

@@ -87,6 +87,9 @@ def test_engine_field():
     with pytest.raises(ValueError):
         native.solve_host(native.ProblemDesc.make(2, 16, 5, 1e-3, 0.1, c, engine=7), 1,
                           np.ones((16, 16)), np.ones((16, 16)))
+    with pytest.raises(ValueError):       # the hybrid engine splits batches; one instance takes one engine
+        native.solve_host(native.ProblemDesc.make(2, 16, 5, 1e-3, 0.1, c, engine="hybrid"), 1,
+                          np.ones((16, 16)), np.ones((16, 16)))
     assert C.sizeof(native.ProblemDesc) == 8 * 4 + 2 * 8 + 23 * 8
 
 

@@ -54,6 +54,55 @@ def oracle_member(oracle, m, steps, bc, dim=2, order=5):
     return oracle.rk4(dim, m.n, order, 0 if bc == "ref" else 1, m.dt, m.dx, steps, m.P, m.coeffs, m.psi0, m.n0)
 
 
+def test_hybrid_members_equal_their_engine_alone(gpu):
+    """The hybrid engine runs the members that fit one workgroup (2-D n <= 64)
+    on the resident engine on a stream of its own and the others on the fused
+    engine: each member equals the same member batched alone on its engine,
+    bit for bit, with step budgets, and so do its FP64 observables.  AUTO
+    picks the hybrid engine for such a mix (same results)."""
+    ms = members_2d([64, 128, 40, 96, 64, 33, 200], seed=7)
+    budgets = [60, 60, 25, 60, 0, 41, 60]
+    mix = Batch(ms, dim=2, order=5, bc="periodic", dtype="f32", engine="hybrid")
+    assert mix.tiles == sum(((m.n + 63) // 64) * ((m.n + 15) // 16) for m in ms if m.n > 64)
+    mix.set_steps(budgets)
+    mix.advance(60)
+    auto = Batch(ms, dim=2, order=5, bc="periodic", dtype="f32", engine="auto")
+    auto.set_steps(budgets)
+    auto.advance(60)
+    obs = mix.observables()
+    for i, m in enumerate(ms):
+        alone = Batch([m], dim=2, order=5, bc="periodic", dtype="f32", engine="resident" if m.n <= 64 else "fused")
+        alone.set_steps([budgets[i]])
+        alone.advance(60)
+        a, an = alone.member(0)
+        b, bn = mix.member(i)
+        assert np.all(np.isfinite(a))
+        np.testing.assert_array_equal(a, b)
+        np.testing.assert_array_equal(an, bn)
+        np.testing.assert_array_equal(auto.member(i)[0], b)
+        np.testing.assert_array_equal(alone.observables()[0], obs[i])
+        if budgets[i] == 0:
+            np.testing.assert_array_equal(b, m.psi0.astype(np.complex64))
+        alone.close()
+    auto.close()
+    mix.close()
+
+
+@pytest.mark.parametrize("bc", ("ref", "periodic"))
+def test_hybrid_1d_f64_matches_oracle(gpu, oracle, bc):
+    """1-D lines of up to 4096 points on the resident engine, longer ones on
+    the per-stage engine, in one hybrid batch."""
+    ms = members_1d([300, 5000, 4096, 6000], ref=(bc == "ref"))
+    b = Batch(ms, dim=1, order=5, bc=bc, dtype="f64", engine="hybrid")
+    b.advance(150)
+    for i, m in enumerate(ms):
+        psi, nres = b.member(i)
+        opsi, onres = oracle.rk4(1, m.n, 5, 0 if bc == "ref" else 1, m.dt, m.dx, 150, m.P, m.coeffs, m.psi0, m.n0)
+        assert rel(psi, opsi) <= TOL64, (i, rel(psi, opsi))
+        assert rel(nres, onres) <= TOL64
+    b.close()
+
+
 @pytest.mark.parametrize("dtype,bc,count,rows", [("f32", "periodic", 20, 32), ("f64", "periodic", 20, 16),
                                                  ("f64", "periodic", 12, 32), ("f64", "ref", 20, 32)])
 @pytest.mark.parametrize("order", (3, 5))
@@ -89,7 +138,7 @@ def test_fused_tile_forms_agree(gpu, oracle, dtype, bc, count, rows, order):
     mix.close()
 
 
-@pytest.mark.parametrize("engine", ("stage", "fused"))
+@pytest.mark.parametrize("engine", ("stage", "fused", "hybrid"))
 @pytest.mark.parametrize("bc", ("periodic", "ref"))
 @pytest.mark.parametrize("dyn", (True, False))
 def test_batch_2d_f64_matches_oracle(gpu, oracle, engine, bc, dyn):
@@ -128,7 +177,7 @@ def test_batch_1d_f64_matches_oracle(gpu, oracle, bc):
         assert rel(psi, opsi) <= TOL64, (i, rel(psi, opsi))
 
 
-@pytest.mark.parametrize("engine", ("stage", "fused"))
+@pytest.mark.parametrize("engine", ("stage", "fused", "hybrid"))
 def test_step_budgets(gpu, oracle, engine):
     ms = members_2d([32, 64, 48, 80], dyn=True)
     budgets = [0, 37, 100, 120]
