@@ -88,6 +88,58 @@ def test_hybrid_members_equal_their_engine_alone(gpu):
     mix.close()
 
 
+RAGGED = [131, 333, 200, 257, 190, 300, 150, 221, 289, 177, 310, 243, 162, 275, 199, 321]
+
+
+@pytest.mark.parametrize("dtype,bc,rows", [("f64", "periodic", 16), ("f64", "ref", 32), ("f32", "periodic", 32)])
+def test_fused_tile_forms_on_ragged_grids(gpu, oracle, dtype, bc, rows):
+    """Grid sides that are not multiples of the tile (131 .. 333) in the forms
+    a launch of many tiles takes on MI355X (572 64 x 32 tiles, more than two
+    rounds of 256 CUs): double precision periodic in 512-thread 64 x 16 tiles,
+    reflecting in 64 x 32 tiles, against the oracle; single precision in
+    64 x 32 tiles against each member alone (64 x 16 tiles), bit for bit."""
+    steps = 20
+    ms = members_2d(RAGGED, seed=50, spacing=0.15)
+    b = Batch(ms, dim=2, order=5, bc=bc, dtype=dtype, engine="fused")
+    assert b.tiles == sum(((n + 63) // 64) * ((n + rows - 1) // rows) for n in RAGGED)
+    b.advance(steps)
+    oracle.set_threads(8)
+    for i in (0, 1, 3, 8, 15):
+        psi, nres = b.member(i)
+        assert np.all(np.isfinite(psi))
+        if dtype == "f64":
+            opsi, onres = oracle_member(oracle, ms[i], steps, bc)
+            assert rel(psi, opsi) <= TOL64, (i, ms[i].n, rel(psi, opsi))
+            assert rel(nres, onres) <= TOL64
+        else:
+            alone = Batch([ms[i]], dim=2, order=5, bc=bc, dtype=dtype, engine="fused")
+            alone.advance(steps)
+            np.testing.assert_array_equal(alone.member(0)[0], psi)
+            np.testing.assert_array_equal(alone.member(0)[1], nres)
+            alone.close()
+    b.close()
+
+
+@pytest.mark.parametrize("sizes,dtype,order,engine", [([32, 64, 48], "f32", 5, "resident"),
+                                                     ([96, 128], "f32", 5, "fused"),
+                                                     ([32, 96], "f64", 7, "fused")])
+def test_hybrid_with_one_part_is_that_engine(gpu, sizes, dtype, order, engine):
+    """A hybrid batch whose members all fit the resident engine, or none do
+    (double precision has no 2-D order-7 resident variant), is a batch of the
+    one engine that holds them: bitwise the same results."""
+    ms = members_2d(sizes, seed=3)
+    a = Batch(ms, dim=2, order=order, bc="periodic", dtype=dtype, engine="hybrid")
+    b = Batch(ms, dim=2, order=order, bc="periodic", dtype=dtype, engine=engine)
+    assert a.tiles == b.tiles
+    a.advance(40)
+    b.advance(40)
+    for i in range(len(ms)):
+        assert np.all(np.isfinite(a.member(i)[0]))
+        np.testing.assert_array_equal(a.member(i)[0], b.member(i)[0])
+    a.close()
+    b.close()
+
+
 @pytest.mark.parametrize("bc", ("ref", "periodic"))
 def test_hybrid_1d_f64_matches_oracle(gpu, oracle, bc):
     """1-D lines of up to 4096 points on the resident engine, longer ones on
