@@ -277,7 +277,15 @@ k_fused_step_2d(const FusedArgs<T> f) {
     const T dt = m.dt;
     const T dt2 = dt * (T)0.5;
     const T dt6 = dt / (T)6;
-    const Coeffs<T> c = m.c;
+    Coeffs<T> c = m.c;
+    if constexpr (sizeof(T) == 4) {
+        // coefficients that meet another coefficient in one fused
+        // multiply-add of the right-hand side (c3 r - c4, d13 + d14 s,
+        // c13 + c14 s): a VALU instruction reads one scalar register, so
+        // held in scalar registers each would be copied to a vector register
+        // per point and stage; kept in vector registers for the whole kernel
+        asm volatile("" : "+v"(c.c4), "+v"(c.d13), "+v"(c.c13));
+    }
     const T w0 = (T)2 * m.w0;          // centre weight of the 2-D Laplacian
     const T w[4] = {m.w0, m.w1, m.w2, m.w3};
     const unsigned *map = ring_map<W, HH, R, WS>.v;
