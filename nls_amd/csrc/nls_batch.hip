@@ -303,9 +303,10 @@ k_fused_step_2d(const FusedArgs<T> f) {
     // points are loaded first, then the fields of all its points, then the
     // LDS stores follow, so that each group of loads is in flight together
     // and waited for once (point by point, each thread waits out the memory
-    // latency once per point: 2.5% slower).  In double precision, with one
-    // workgroup per CU, point by point is 7% faster and is kept.
-    constexpr bool GROUPED = sizeof(T) == 4;
+    // latency once per point: 2.5% slower).  In double precision the 512-thread
+    // form, two workgroups per CU, groups its loads too (4% at 2048^2); with
+    // one 1024-thread workgroup per CU point by point is 7% faster and is kept.
+    constexpr bool GROUPED = sizeof(T) == 4 || FORM == FORM_HALF;
     unsigned ent[J];
 #pragma unroll
     for (int j = 0; j < J; ++j) {
