@@ -7,16 +7,21 @@
 // budget is used up passes its state through unchanged, so a batch can hold
 // solves of different lengths (the serving loop relies on this).
 //
-// Two engines:
-//   NLS_ENGINE_STAGE  one launch per RK4 stage (bodies in nls_stage.h), 15 S
-//                     + 4 r bytes per point and step; any dim/boundary/order.
-//   NLS_ENGINE_FUSED  one launch per RK4 step, 2-D only.  A workgroup loads
-//                     its output tile plus a halo of 4R points (R = stencil
-//                     radius) into registers and LDS once, runs the four
-//                     stages on the shrinking region on chip, and writes the
-//                     new state of the tile: 2 S + r bytes per point and step
-//                     from HBM plus the halo re-reads, at the price of
-//                     recomputing the halo's stages.
+// Engines:
+//   NLS_ENGINE_STAGE    one launch per RK4 stage (bodies in nls_stage.h), 15 S
+//                       + 4 r bytes per point and step; any dim/boundary/order.
+//   NLS_ENGINE_FUSED    one launch per RK4 step, 2-D only.  A workgroup loads
+//                       its output tile plus a halo of 4R points (R = stencil
+//                       radius) into registers and LDS once, runs the four
+//                       stages on the shrinking region on chip, and writes the
+//                       new state of the tile: 2 S + r bytes per point and
+//                       step from HBM plus the halo re-reads, at the price of
+//                       recomputing the halo's stages.
+//   NLS_ENGINE_RESIDENT a workgroup per member holds the member on chip and
+//                       runs every step of a call (1-D n <= 4096, 2-D n <= 64).
+//   NLS_ENGINE_HYBRID   the members that fit the resident engine on it, on a
+//                       stream of the batch, beside the others on the fused
+//                       (2-D) or per-stage (1-D) engine.
 #include <algorithm>
 #include <atomic>
 #include <cstring>
