@@ -959,6 +959,7 @@ struct nls_batch {
     size_t points;                 // length of the packed arrays
     int ntiles, chunk;
     void *d_mem = nullptr, *d_tiles = nullptr, *d_wtab = nullptr;
+    void *d_obs = nullptr;              // observables of every member (4 doubles each), allocated at first use
     void *h_steps_pinned = nullptr;
     std::vector<unsigned char> h_mem;   // host image of the member table
     size_t mem_stride;
@@ -1341,6 +1342,7 @@ int nls_batch_destroy(nls_batch *b) {
     if (b->d_mem) (void)hipFree(b->d_mem);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     if (b->d_wtab) (void)hipFree(b->d_wtab);
+    if (b->d_obs) (void)hipFree(b->d_obs);
     delete b;
     return NLS_OK;
 }
@@ -1400,9 +1402,10 @@ int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, doubl
     }
     hipStream_t s = (hipStream_t)stream;
     const int radial = (b->dim == 1 && b->bc == NLS_BC_REF) ? 1 : 0;
-    void *d_out = nullptr;
-    int rc = hip_check(hipMalloc(&d_out, (size_t)b->count * 4 * sizeof(double)), "observables buffer");
+    int rc = NLS_OK;
+    if (!b->d_obs) rc = hip_check(hipMalloc(&b->d_obs, (size_t)b->count * 4 * sizeof(double)), "observables buffer");
     if (rc) return rc;
+    void *d_out = b->d_obs;
     if (b->dtype == NLS_F64)
         hipLaunchKernelGGL((k_batch_obs<double>), dim3(b->count), dim3(256), 0, s, (const MemberDev<double> *)b->d_mem,
                            b->dim, radial, (const double2 *)psi, b->dyn ? (const double *)nres : nullptr,
@@ -1415,7 +1418,6 @@ int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, doubl
     if (!rc) rc = hip_check(hipMemcpyAsync(out, d_out, (size_t)b->count * 4 * sizeof(double), hipMemcpyDeviceToHost, s),
                             "observables download");
     if (!rc) rc = hip_check(hipStreamSynchronize(s), "observables sync");
-    (void)hipFree(d_out);
     if (rc) return rc;
     for (int i = 0; i < b->count; ++i) {
         // volume element, as nls_observables: 2 pi r dr on the radial grid
