@@ -133,11 +133,13 @@ def batch_members(dtype_dim=2):
     return out
 
 
-@pytest.mark.parametrize("engine", ("fused", "stage"))
+@pytest.mark.parametrize("engine", ("fused", "stage", "hybrid"))
 @pytest.mark.parametrize("dtype", ("f64", "f32"))
 def test_batch_graph_replays_equal_direct_launches(gpu, engine, dtype):
     """A batch replays a captured 64-step block when every member's budget
-    covers the call: bitwise equal to direct launches, on every replay."""
+    covers the call: bitwise equal to direct launches, on every replay.  A
+    hybrid batch (48^2 and 64^2 resident, 80^2 fused) passes the graph
+    setting to its fused part."""
     import torch
     from nls_amd.batch import Batch
     ms = batch_members()
@@ -161,7 +163,7 @@ def test_batch_graph_replays_equal_direct_launches(gpu, engine, dtype):
     g.close()
 
 
-@pytest.mark.parametrize("engine", ("fused", "stage"))
+@pytest.mark.parametrize("engine", ("fused", "stage", "hybrid"))
 def test_batch_budgets_inside_a_call_are_honoured(gpu, engine):
     """Budgets that end inside a call: the launches are issued directly, the
     step index of each is exact, and every member stops at its budget."""
