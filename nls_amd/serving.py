@@ -19,7 +19,9 @@ schedulr/workr.py:39-107).  Here requests share the GPU:
   min(block, remaining) steps (nls_batch_set_steps), so a request that
   finishes mid-block holds its state and its slot is freed at the end of the
   block, when waiting requests are admitted; the block length adapts so one
-  block takes about target_block_ms;
+  block takes about target_block_ms (20 ms), or idle_block_ms (5 ms) while
+  the load is light (none waiting, at most idle_max_active in flight), so a
+  request waits less for block boundaries;
 * each pool runs on its own stream (default priority), so the 1-D lines,
   small grids and large grids of a block run on the device at the same time;
 * a member's arithmetic never depends on the other members, so a result is
@@ -233,7 +235,7 @@ class Engine(object):
     SHARES = {"line": 0.15, "small": 0.15, "large": 0.70}
 
     def __init__(self, budget_bytes=8 << 30, dtype="f32", device=None, target_block_ms=20.0,
-                 min_block=8, max_block=2000):
+                 min_block=8, max_block=2000, idle_block_ms=5.0, idle_max_active=32):
         import torch
         self.torch = torch
         self.device = device or torch.device("cuda:0")
@@ -244,6 +246,11 @@ class Engine(object):
             cap = int(budget_bytes * self.SHARES[key] // Group.bytes_per_point(eng, dtype)) // ALIGN * ALIGN
             self.groups[key] = Group(dim, cap, dtype, self.device, eng)
         self.target_block_ms, self.min_block, self.max_block = target_block_ms, min_block, max_block
+        # shorter blocks while the load is light (no request waits for
+        # admission and at most idle_max_active are in flight): a request
+        # admitted or finished mid-block waits less for the block's end
+        self.idle_block_ms = target_block_ms if idle_block_ms is None else idle_block_ms
+        self.idle_max_active = idle_max_active
         self.block = min_block
         self.blocks = 0
 
@@ -284,7 +291,9 @@ class Engine(object):
             self.blocks += 1
             el = (time.perf_counter() - tb) * 1e3
             per_step = el / max(self.block, 1)
-            self.block = int(min(self.max_block, max(self.min_block, self.target_block_ms / max(per_step, 1e-4))))
+            light = not waiting and sum(len(g.members) for g in active) <= self.idle_max_active
+            target = self.idle_block_ms if light else self.target_block_ms
+            self.block = int(min(self.max_block, max(self.min_block, target / max(per_step, 1e-4))))
             if log and self.blocks % 50 == 0:
                 log("t=%.2fs finished=%d waiting=%d active=%d block=%d" % (
                     now(), len(finished), len(waiting), sum(len(g.members) for g in self.groups.values()),
