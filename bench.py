@@ -212,7 +212,9 @@ def config_accuracy(dev):
     after the full step count of each configuration: line256 (1-D N=256,
     2000 steps), square256 (256^2, 10^4 steps), square2048 (2048^2, 2*10^4
     steps, RK4 and split-step Fourier).  The FP64 GPU path is the reference
-    here; the tests pin it to the FP64 oracle (relL2 <= 1e-10)."""
+    here; the tests pin it to the FP64 oracle (relL2 <= 1e-10).  Beside it,
+    the FP64-accumulated norm and energy (nls_observables, nls_energy) of
+    both final states."""
     import numpy as np
     import torch
     from nls_amd import configs, native
@@ -220,7 +222,7 @@ def config_accuracy(dev):
     cases = [("line256", configs.line256(), "rk4"), ("square256", configs.square256(), "rk4"),
              ("square2048_rk4", configs.square2048(), "rk4"), ("square2048_ssf", configs.square2048(), "ssf")]
     for name, inst, scheme in cases:
-        res = {}
+        res, obs = {}, {}
         for dtype in ("f64", "f32"):
             d = native.ProblemDesc.make(inst.dim, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic",
                                         dtype=dtype, scheme=scheme)
@@ -233,9 +235,15 @@ def config_accuracy(dev):
             native.advance(d, inst.steps, P, psi, nres, ws)
             torch.cuda.synchronize()
             res[dtype] = psi.cpu().numpy().astype(np.complex128)
+            obs[dtype] = {"norm": native.observables(d, psi, nres)["norm"],
+                          "energy": native.energy(d, P, psi, nres)["energy"]}
         err = float(np.linalg.norm(res["f32"] - res["f64"]) / np.linalg.norm(res["f64"]))
-        out[name] = {"relL2_f32_vs_f64": err, "steps": inst.steps, "points": inst.points}
-        log("accuracy %s (%d steps): FP32 vs FP64 relL2 = %.2e" % (name, inst.steps, err))
+        out[name] = {"relL2_f32_vs_f64": err, "steps": inst.steps, "points": inst.points,
+                     "fp64_accumulated": {"norm_f64": obs["f64"]["norm"], "norm_f32": obs["f32"]["norm"],
+                                          "energy_f64": obs["f64"]["energy"], "energy_f32": obs["f32"]["energy"]}}
+        log("accuracy %s (%d steps): FP32 vs FP64 relL2 = %.2e; norm %.10g / %.10g, energy %.10g / %.10g "
+            "(FP64 / FP32 state, FP64 sums)" % (name, inst.steps, err, obs["f64"]["norm"], obs["f32"]["norm"],
+                                               obs["f64"]["energy"], obs["f32"]["energy"]))
     return out
 
 

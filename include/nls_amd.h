@@ -232,6 +232,18 @@ int nls_chemical_potential(const nls_problem *p, const void *pumping, const void
  * Blocks until done. */
 int nls_observables(const nls_problem *p, const void *psi, const void *nres, double *out, void *stream);
 
+/* Energy of a device state for the conservative part of the right-hand side
+ * (nls.f90:637-644: the operator -L + c5|psi|^2 + c6 r), double-precision
+ * accumulators in a fixed order: out[1] = kinetic -Re<psi, L psi> dV with the
+ * problem's own discrete Laplacian L, out[2] = interaction (c5/2) sum |psi|^4
+ * dV, out[3] = reservoir coupling c6 sum r |psi|^2 dV (r = n for the dynamic
+ * reservoir, else c12 P / (c13 + c14 |psi|^2)), out[0] = their sum; dV as
+ * nls_observables (2*pi*r*dr with the weight r on the radial grid).  The
+ * reference defines no energy; its chemical potential is the same operator's
+ * expectation, Re mu * norm = out[1] + 2 out[2] + out[3].  Blocks until done. */
+int nls_energy(const nls_problem *p, const void *pumping, const void *psi, const void *nres, double *out,
+               void *stream);
+
 /* ======================================================================
  * Input fields generated on the device: pumping profiles and initial
  * states sampled on a grid, so inputs given by parameters need no

@@ -4,7 +4,11 @@
 //   - the chemical potential mu = i <u, F(u)>_w / <u, u>_w of
 //     chemical_potential_1d/2d (reference nls/nls.f90:921-971): weight
 //     w_k = k*dx on the 1-D radial grid (nls.f90:941-947), 1 in 2-D;
-//   - norm, reservoir population, peak density and a non-finite count.
+//   - norm, reservoir population, peak density and a non-finite count;
+//   - the energy of the conservative part of the right-hand side,
+//     E = -Re<psi, L psi>_w + (c5/2) sum |psi|^4 w + c6 sum r |psi|^2 w
+//     (times the volume element), whose variation at fixed reservoir r gives
+//     the operator -L + c5|psi|^2 + c6 r of nls.f90:637-644.
 // Pass 1: each 256-thread workgroup reduces a grid-strided slice in a fixed
 // tree; pass 2: one workgroup sums the per-workgroup partials in order.
 #include <cstring>
@@ -76,6 +80,25 @@ __global__ void __launch_bounds__(256) k_obs_partial(size_t N, const cplx_t<T> *
         __syncthreads();
     }
     if (threadIdx.x == 0) part[(size_t)blockIdx.x * RED_VALS + 2] = sh[0][0];
+}
+
+// sum |psi|^4 w and sum r |psi|^2 w, r the reservoir: n (dynamic) or the
+// algebraic c12 P / (c13 + c14 |psi|^2) (nls.f90:570-581)
+template <typename T>
+__global__ void __launch_bounds__(256) k_energy_partial(size_t N, const cplx_t<T> *psi, const T *P, const T *nres,
+                                                        double c12, double c13, double c14, int radial, double dx,
+                                                        double *part) {
+    __shared__ double sh[RED_VALS][256];
+    double v[RED_VALS] = {0, 0, 0, 0};
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < N; i += (size_t)gridDim.x * 256) {
+        const double w = radial ? (double)i * dx : 1.0;
+        const double pr = psi[i].x, pi = psi[i].y;
+        const double s = pr * pr + pi * pi;
+        const double r = nres ? (double)nres[i] : c12 * (double)P[i] / (c13 + c14 * s);
+        v[0] += s * s * w;
+        v[1] += r * s * w;
+    }
+    block_sum(sh, v, part + (size_t)blockIdx.x * RED_VALS, 2);
 }
 
 __global__ void __launch_bounds__(256) k_sum_partials(int nb, const double *part, int max_slot, double *out) {
@@ -154,6 +177,58 @@ static int chemical_potential_dev(const nls_problem *p, const void *P, const voi
     return NLS_OK;
 }
 
+// Energy terms on device buffers.  The kinetic term comes from the
+// right-hand side kernels with every term but the Laplacian switched off
+// (c3 = c4 = c5 = c6 = c12 = c14 = 0, c13 = 1: F = i L psi), so it uses the
+// solver's own discrete operator; then -Re<psi, L psi>_w = -Im<psi, F>_w.
+static int energy_dev(const nls_problem *p, const void *P, const void *psi, const void *nres, double *out,
+                      hipStream_t s) {
+    const size_t N = points_of(p), rs = p->dtype == NLS_F64 ? 8 : 4;
+    const bool dyn = p->coeffs[10] != 0.0;
+    nls_problem q = *p;
+    q.coeffs[2] = q.coeffs[3] = q.coeffs[4] = q.coeffs[5] = 0.0;
+    q.coeffs[10] = 0.0; q.coeffs[11] = 0.0; q.coeffs[12] = 1.0; q.coeffs[13] = 0.0;
+    void *scratch = nullptr;
+    const size_t fbytes = N * 2 * rs;
+    const size_t red = (size_t)RED_BLOCKS * RED_VALS * sizeof(double) + 2 * RED_VALS * sizeof(double);
+    int rc = hip_check(hipMalloc(&scratch, fbytes + red + 512), "energy scratch");
+    if (rc) return rc;
+    char *base = (char *)scratch;
+    void *F = base;
+    double *part = (double *)(base + ((fbytes + 255) & ~(size_t)255));
+    double *outK = part + RED_BLOCKS * RED_VALS, *outV = outK + RED_VALS;
+    rc = nls_rhs(&q, P, psi, nullptr, F, nullptr, s);
+    const int radial = (p->dim == 1 && p->bc == NLS_BC_REF) ? 1 : 0;
+    const int nb = red_blocks(N);
+    if (!rc) {
+        if (p->dtype == NLS_F64) {
+            cdot<double>(N, psi, F, radial, p->dx, part, outK, s);
+            hipLaunchKernelGGL((k_energy_partial<double>), dim3(nb), dim3(256), 0, s, N, (const double2 *)psi,
+                               (const double *)P, dyn ? (const double *)nres : nullptr, p->coeffs[11], p->coeffs[12],
+                               p->coeffs[13], radial, p->dx, part);
+        } else {
+            cdot<float>(N, psi, F, radial, p->dx, part, outK, s);
+            hipLaunchKernelGGL((k_energy_partial<float>), dim3(nb), dim3(256), 0, s, N, (const float2 *)psi,
+                               (const float *)P, dyn ? (const float *)nres : nullptr, p->coeffs[11], p->coeffs[12],
+                               p->coeffs[13], radial, p->dx, part);
+        }
+        hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(256), 0, s, nb, part, -1, outV);
+        rc = hip_check(hipGetLastError(), "energy reduction");
+    }
+    double hK[RED_VALS], hV[RED_VALS];
+    if (!rc) rc = hip_check(hipMemcpyAsync(hK, outK, sizeof hK, hipMemcpyDeviceToHost, s), "energy download");
+    if (!rc) rc = hip_check(hipMemcpyAsync(hV, outV, sizeof hV, hipMemcpyDeviceToHost, s), "energy download");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "energy sync");
+    (void)hipFree(scratch);
+    if (rc) return rc;
+    const double dV = radial ? 2.0 * M_PI * p->dx : (p->dim == 1 ? p->dx : p->dx * p->dx);
+    out[1] = -hK[1] * dV;                               // -Re<psi, L psi> = -Im<psi, i L psi>
+    out[2] = 0.5 * p->coeffs[4] * hV[0] * dV;
+    out[3] = p->coeffs[5] * hV[1] * dV;
+    out[0] = out[1] + out[2] + out[3];
+    return NLS_OK;
+}
+
 template <typename T>
 static int observables_dev(const nls_problem *p, const void *psi, const void *nres, double *out, hipStream_t s) {
     const size_t N = points_of(p);
@@ -194,6 +269,15 @@ int nls_chemical_potential(const nls_problem *p, const void *pumping, const void
     if (rc) return rc;
     if (!pumping || !psi || !mu || (p->coeffs[10] != 0.0 && !nres)) return fail(NLS_EINVAL, "NULL argument");
     return chemical_potential_dev(p, pumping, psi, nres, mu, (hipStream_t)stream);
+}
+
+int nls_energy(const nls_problem *p, const void *pumping, const void *psi, const void *nres, double *out,
+               void *stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (!pumping || !psi || !out || (p->coeffs[10] != 0.0 && !nres)) return fail(NLS_EINVAL, "NULL argument");
+    if (p->scheme != NLS_SCHEME_RK4 && p->scheme != NLS_SCHEME_SSF) return fail(NLS_EINVAL, "bad scheme");
+    return energy_dev(p, pumping, psi, nres, out, (hipStream_t)stream);
 }
 
 int nls_observables(const nls_problem *p, const void *psi, const void *nres, double *out, void *stream) {

@@ -30,7 +30,7 @@ EXPORTS = (
     "nls_last_error", "nls_version", "nls_solve_nls", "nls_solve_nls_1d", "nls_solve_nls_2d",
     "nls_chemical_potential_1d", "nls_chemical_potential_2d", "nls_make_laplacian",
     "nls_solve_host", "nls_workspace_bytes", "nls_advance", "nls_rhs",
-    "nls_chemical_potential", "nls_observables",
+    "nls_chemical_potential", "nls_observables", "nls_energy",
     "nls_batch_create", "nls_batch_destroy", "nls_batch_points", "nls_batch_tiles",
     "nls_batch_workspace_bytes", "nls_batch_set_steps", "nls_batch_advance", "nls_tune_stage_tile",
     "nls_release", "nls_fill_profile", "nls_fill_state", "nls_slab_halo_rows", "nls_slab_step",
@@ -155,6 +155,7 @@ def load():
         "nls_rhs": ([pp, vp, vp, vp, vp, vp, vp], i),
         "nls_chemical_potential": ([pp, vp, vp, vp, vp, vp], i),
         "nls_observables": ([pp, vp, vp, vp, vp], i),
+        "nls_energy": ([pp, vp, vp, vp, vp, vp], i),
         "nls_batch_create": ([i, i, i, i, i, C.POINTER(Member), i, C.POINTER(vp)], i),
         "nls_batch_destroy": ([vp], i),
         "nls_batch_points": ([vp], sz),
@@ -514,6 +515,17 @@ def chemical_potential(desc, pumping, psi, nres=None, stream=None):
     _check(load().nls_chemical_potential(C.byref(desc), _dp(pumping), _dp(psi), _dp(nres), mu,
                                          C.c_void_p(s.cuda_stream)))
     return complex(mu[0], mu[1])
+
+
+def energy(desc, pumping, psi, nres=None, stream=None):
+    """Energy of a device state for the conservative part of the right-hand
+    side (nls_energy): {"energy", "kinetic", "interaction", "reservoir"},
+    accumulated in double precision."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    out = (C.c_double * 4)()
+    _check(load().nls_energy(C.byref(desc), _dp(pumping), _dp(psi), _dp(nres), out, C.c_void_p(s.cuda_stream)))
+    return {"energy": out[0], "kinetic": out[1], "interaction": out[2], "reservoir": out[3]}
 
 
 def observables(desc, psi, nres=None, stream=None):

@@ -1,0 +1,137 @@
+"""nls_energy: the energy of the conservative part of the right-hand side,
+E = -Re<psi, L psi> dV + (c5/2) sum |psi|^4 dV + c6 sum r |psi|^2 dV,
+accumulated in double precision.
+
+Pins: a plane wave on a periodic grid, whose kinetic term is the stencil's
+eigenvalue in closed form; the identity Re(mu) * norm = kinetic +
+2 interaction + reservoir with the chemical potential (nls_chemical_potential,
+an independent path through the full right-hand side) and the norm
+(nls_observables); the oracle's Laplacian for the kinetic term on the
+reference's boundaries (radial 1-D, zero outside the square)."""
+import itertools
+
+import numpy as np
+import pytest
+
+from nls_amd import configs, native
+
+pytestmark = pytest.mark.gpu
+
+
+def eig(order, theta):
+    """h^2 times the eigenvalue of the order-`order` second difference on exp(i theta j)."""
+    c = np.cos
+    if order == 3:
+        return -4.0 * np.sin(theta / 2) ** 2
+    if order == 5:
+        return (-2 * c(2 * theta) + 32 * c(theta) - 30) / 12.0
+    return (4 * c(3 * theta) - 54 * c(2 * theta) + 540 * c(theta) - 490) / 180.0
+
+
+def coeffs(dyn):
+    c = configs.model_coeffs(dynamic=dyn)
+    c[4], c[5] = 1.3, 0.7
+    return c
+
+
+def dev(a, dtype):
+    import torch
+    if a is None:
+        return None
+    t = {("f64", True): torch.complex128, ("f64", False): torch.float64,
+         ("f32", True): torch.complex64, ("f32", False): torch.float32}[(dtype, np.iscomplexobj(a))]
+    return torch.as_tensor(np.ascontiguousarray(a)).to(t).cuda()
+
+
+@pytest.mark.parametrize("dim,order,dyn", list(itertools.product((1, 2), (3, 5, 7), (True, False))))
+def test_plane_wave_closed_form(gpu, dim, order, dyn):
+    n, h, A = 64, 0.25, 0.7 + 0.2j
+    m = (5, 3)
+    j = np.arange(n)
+    if dim == 1:
+        psi = A * np.exp(2j * np.pi * m[0] * j / n)
+        lam = eig(order, 2 * np.pi * m[0] / n) / h ** 2
+    else:
+        J, K = np.meshgrid(j, j, indexing="ij")
+        psi = A * np.exp(2j * np.pi * (m[0] * J + m[1] * K) / n)
+        lam = (eig(order, 2 * np.pi * m[0] / n) + eig(order, 2 * np.pi * m[1] / n)) / h ** 2
+    c = coeffs(dyn)
+    P = np.full(psi.shape, 2.5)
+    n0 = np.full(psi.shape, 0.4) if dyn else None
+    d = native.ProblemDesc.make(dim, n, order, 1e-3, h, c, bc="periodic", dtype="f64")
+    e = native.energy(d, dev(P, "f64"), dev(psi, "f64"), dev(n0, "f64"))
+    vol = (n * h) ** dim
+    s = abs(A) ** 2
+    r = 0.4 if dyn else c[11] * 2.5 / (c[12] + c[13] * s)
+    want = {"kinetic": -lam * s * vol, "interaction": 0.5 * c[4] * s * s * vol, "reservoir": c[5] * r * s * vol}
+    for k, v in want.items():
+        assert abs(e[k] - v) <= 1e-12 * abs(v), (k, e[k], v)
+    assert abs(e["energy"] - sum(want.values())) <= 1e-12 * abs(sum(want.values()))
+
+
+def smooth_case(dim, n, bc, dyn, seed):
+    rng = np.random.default_rng(seed)
+    if dim == 1:
+        h = 0.1
+        x = np.arange(n) * h if bc == "ref" else configs.grid_1d(n, n * h)[0]
+        r2 = x ** 2
+    else:
+        X, Y, h = configs.grid_2d(n, 0.2 * n)
+        r2 = X ** 2 + Y ** 2
+    psi = (0.5 + 0.2j) * np.exp(-r2 / 6.0) + configs.noise(rng, r2.shape, 1e-2)
+    P = 2.0 * np.exp(-r2 / 9.0)
+    n0 = 0.3 * np.exp(-r2 / 4.0) if dyn else None
+    return h, psi, P, n0
+
+
+@pytest.mark.parametrize("dim,bc,order,dyn", [(d, b, o, y) for d, b, o, y in
+                                              itertools.product((1, 2), ("ref", "periodic"), (3, 5, 7), (True, False))
+                                              ])
+def test_identity_with_chemical_potential(gpu, dim, bc, order, dyn):
+    n = 96 if dim == 1 else 40
+    h, psi, P, n0 = smooth_case(dim, n, bc, dyn, seed=order)
+    c = coeffs(dyn)
+    d = native.ProblemDesc.make(dim, n, order, 1e-3, h, c, bc=bc, dtype="f64")
+    Pd, psid, nd = dev(P, "f64"), dev(psi, "f64"), dev(n0, "f64")
+    e = native.energy(d, Pd, psid, nd)
+    mu = native.chemical_potential(d, Pd, psid, nd)
+    norm = native.observables(d, psid, nd)["norm"]
+    lhs = mu.real * norm
+    rhs = e["kinetic"] + 2 * e["interaction"] + e["reservoir"]
+    assert abs(lhs - rhs) <= 1e-11 * (abs(e["kinetic"]) + abs(e["interaction"]) + abs(e["reservoir"])), (lhs, rhs)
+    assert e["kinetic"] > 0
+
+
+@pytest.mark.parametrize("dim,bc,order", [(1, "ref", 5), (1, "ref", 3), (2, "ref", 5), (2, "periodic", 7)])
+def test_kinetic_term_matches_oracle(gpu, oracle, dim, bc, order):
+    n = 80 if dim == 1 else 32
+    h, psi, P, _ = smooth_case(dim, n, bc, False, seed=11)
+    c = np.zeros(23)
+    c[12] = 1.0                                     # F = i L psi: every other term off
+    F, _ = oracle.rhs(dim, n, order, 0 if bc == "ref" else 1, h, np.zeros(psi.shape), c, psi)
+    Lpsi = -1j * F
+    if dim == 1 and bc == "ref":
+        w, dV = np.arange(n) * h, 2 * np.pi * h
+    else:
+        w, dV = 1.0, h ** dim
+    want = -np.sum((np.conj(psi) * Lpsi).real * w) * dV
+    d = native.ProblemDesc.make(dim, n, order, 1e-3, h, coeffs(False), bc=bc, dtype="f64")
+    e = native.energy(d, dev(P, "f64"), dev(psi, "f64"))
+    assert abs(e["kinetic"] - want) <= 1e-11 * abs(want), (e["kinetic"], want)
+
+
+def test_single_precision_state_accumulates_in_double(gpu):
+    """An FP32 state gives the energy of its FP64 copy to double-precision
+    accumulation: only the products of each point are formed from the
+    rounded state (the Laplacian in single precision), so the two agree to
+    the state's rounding."""
+    h, psi, P, n0 = smooth_case(2, 64, "periodic", True, seed=2)
+    psi32 = psi.astype(np.complex64)
+    c = coeffs(True)
+    e32 = native.energy(native.ProblemDesc.make(2, 64, 5, 1e-3, h, c, bc="periodic", dtype="f32"),
+                        dev(P, "f32"), dev(psi32, "f32"), dev(n0, "f32"))
+    e64 = native.energy(native.ProblemDesc.make(2, 64, 5, 1e-3, h, c, bc="periodic", dtype="f64"),
+                        dev(P.astype(np.float32), "f64"), dev(psi32.astype(np.complex128), "f64"),
+                        dev(n0.astype(np.float32), "f64"))
+    for k in ("kinetic", "interaction", "reservoir"):
+        assert abs(e32[k] - e64[k]) <= 1e-5 * abs(e64[k]), (k, e32[k], e64[k])
