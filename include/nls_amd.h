@@ -344,6 +344,13 @@ int nls_batch_prepare(nls_batch *b, int iters, const void *pumping, void *psi, v
  * of non-finite points of member i.  out is a host array of 4 count
  * doubles; blocks until done. */
 int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, double *out, void *stream);
+/* Energy of every member (nls_energy of each member alone, with the member's
+ * own stencil weights): out[4 i + 0..3] = energy, kinetic, interaction,
+ * reservoir coupling of member i, FP64 accumulators in a fixed order per
+ * member; out is a host array of 4 count doubles; blocks until done.  No
+ * counterpart in the reference. */
+int nls_batch_energy(nls_batch *b, const void *pumping, const void *psi, const void *nres, double *out,
+                     void *stream);
 
 #ifdef __cplusplus
 }

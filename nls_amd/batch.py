@@ -109,6 +109,19 @@ class Batch(object):
             out.ctypes.data_as(C.c_void_p), C.c_void_p(s.cuda_stream)))
         return out
 
+    def energy(self, stream=None):
+        """Per-member (energy, kinetic, interaction, reservoir coupling) of the
+        conservative part of the right-hand side, accumulated in double
+        precision on the device (nls_batch_energy): a (count, 4) float64
+        array."""
+        s = stream if stream is not None else self.torch.cuda.current_stream()
+        out = np.zeros((self.count, 4), np.float64)
+        native._check(native.load().nls_batch_energy(
+            self.handle, C.c_void_p(self.P.data_ptr()), C.c_void_p(self.psi.data_ptr()),
+            C.c_void_p(self.nres.data_ptr()) if self.nres is not None else None,
+            out.ctypes.data_as(C.c_void_p), C.c_void_p(s.cuda_stream)))
+        return out
+
     def member(self, i):
         """(psi, n) of member i as host numpy arrays in its grid shape."""
         a, b = self.offsets[i], self.offsets[i] + self.sizes[i]

@@ -1135,6 +1135,73 @@ __global__ void __launch_bounds__(256) k_batch_obs(const MemberDev<T> *mem, int 
     }
 }
 
+// Energy terms of every member (nls_energy per member): a workgroup per
+// member, the Laplacian of each point from its neighbours in the packed
+// array (the resident kernel's stencil: the member's weights, its radial
+// table on the 1-D reference grid), FP64 accumulators, wavefront shuffles and
+// the four wavefronts added in order.  out[4 i + 1..3] = kinetic
+// -Re(conj psi L psi) w, (c5/2) |psi|^4 w, c6 r |psi|^2 w summed (w = i on the
+// radial grid, else 1; the host applies the volume element).
+template <typename T, int ORDER, int BC, int DIM>
+__global__ void __launch_bounds__(256) k_batch_energy(const MemberDev<T> *mem, const T *wtab,
+                                                      const cplx_t<T> *psi, const T *P, const T *nres,
+                                                      double *out) {
+    __shared__ double part[4][3];
+    const MemberDev<T> m = mem[blockIdx.x];
+    const int n = m.n;
+    const long long N = DIM == 1 ? (long long)n : (long long)n * n;
+    const cplx_t<T> *u = psi + m.off;
+    const T *pp = P + m.off;
+    const T *nn = nres ? nres + m.off : nullptr;
+    const T *wt = (DIM == 1 && BC == BC_REF) ? wtab + m.tab : nullptr;
+    const T w[4] = {m.w0, m.w1, m.w2, m.w3};
+    const double c12 = m.c.c12, c13 = m.c.c13, c14 = m.c.c14;
+    double v[3] = {0, 0, 0};
+    for (long long i = threadIdx.x; i < N; i += 256) {
+        const double wi = (DIM == 1 && BC == BC_REF) ? (double)i : 1.0;
+        const cplx_t<T> lap = resident_lap<T, ORDER, BC, DIM>(u, (int)i, n, w, wt ? wt + (size_t)i * ORDER : nullptr);
+        const double pr = u[i].x, pi = u[i].y;
+        const double sq = pr * pr + pi * pi;
+        const double r = nn ? (double)nn[i] : c12 * (double)pp[i] / (c13 + c14 * sq);
+        v[0] -= (pr * (double)lap.x + pi * (double)lap.y) * wi;
+        v[1] += sq * sq * wi;
+        v[2] += r * sq * wi;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+        for (int q = 0; q < 3; ++q) v[q] += __shfl_xor(v[q], off, 64);
+    const int wave = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0)
+        for (int q = 0; q < 3; ++q) part[wave][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double o[3] = {part[0][0], part[0][1], part[0][2]};
+        for (int k = 1; k < 4; ++k)
+            for (int q = 0; q < 3; ++q) o[q] += part[k][q];
+        out[(size_t)blockIdx.x * 4 + 0] = 0.0;
+        out[(size_t)blockIdx.x * 4 + 1] = o[0];
+        out[(size_t)blockIdx.x * 4 + 2] = 0.5 * (double)m.c.c5 * o[1];
+        out[(size_t)blockIdx.x * 4 + 3] = (double)m.c.c6 * o[2];
+    }
+}
+
+template <typename T>
+static void launch_batch_energy(const nls_batch *b, const T *P, const cplx_t<T> *psi, const T *nres, double *out,
+                                hipStream_t s) {
+    const MemberDev<T> *mem = (const MemberDev<T> *)b->d_mem;
+    const T *wt = (const T *)b->d_wtab;
+    const dim3 grid(b->count), blk(256);
+#define NLS_CASE(O, B, D)                                                                                     \
+    if (b->order == O && b->bc == B && b->dim == D) {                                                          \
+        hipLaunchKernelGGL((k_batch_energy<T, O, B, D>), grid, blk, 0, s, mem, wt, psi, P, nres, out);         \
+        return;                                                                                               \
+    }
+    NLS_CASE(3, 0, 1) NLS_CASE(3, 1, 1) NLS_CASE(3, 0, 2) NLS_CASE(3, 1, 2)
+    NLS_CASE(5, 0, 1) NLS_CASE(5, 1, 1) NLS_CASE(5, 0, 2) NLS_CASE(5, 1, 2)
+    NLS_CASE(7, 0, 1) NLS_CASE(7, 1, 1) NLS_CASE(7, 0, 2) NLS_CASE(7, 1, 2)
+#undef NLS_CASE
+}
+
 static int drop_graph(nls_batch *b) {
     if (!b->graph) return NLS_OK;
     // an earlier launch of it may still be in flight on any stream
@@ -1426,6 +1493,46 @@ int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, doubl
         const double dV = radial ? 2.0 * M_PI * dx * dx : (b->dim == 1 ? dx : dx * dx);
         out[4 * i + 0] *= dV;
         out[4 * i + 1] *= dV;
+    }
+    return NLS_OK;
+}
+
+int nls_batch_energy(nls_batch *b, const void *pumping, const void *psi, const void *nres, double *out,
+                     void *stream) {
+    if (!b || !pumping || !psi || !out) return fail(NLS_EINVAL, "NULL argument");
+    if (b->dyn && !nres) return fail(NLS_EINVAL, "NULL reservoir buffer");
+    if (is_hybrid(b)) {
+        for (int k = 0; k < 2; ++k) {
+            std::vector<double> o(b->ids[k].size() * 4);
+            const int rc = nls_batch_energy(b->part[k], pumping, psi, nres, o.data(), stream);
+            if (rc) return rc;
+            for (size_t q = 0; q < b->ids[k].size(); ++q)
+                for (int c = 0; c < 4; ++c) out[4 * b->ids[k][q] + c] = o[4 * q + c];
+        }
+        return NLS_OK;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    int rc = NLS_OK;
+    if (!b->d_obs) rc = hip_check(hipMalloc(&b->d_obs, (size_t)b->count * 4 * sizeof(double)), "observables buffer");
+    if (rc) return rc;
+    double *d_out = (double *)b->d_obs;
+    if (b->dtype == NLS_F64)
+        launch_batch_energy<double>(b, (const double *)pumping, (const double2 *)psi,
+                                    b->dyn ? (const double *)nres : nullptr, d_out, s);
+    else
+        launch_batch_energy<float>(b, (const float *)pumping, (const float2 *)psi,
+                                   b->dyn ? (const float *)nres : nullptr, d_out, s);
+    rc = hip_check(hipGetLastError(), "energy launch");
+    if (!rc) rc = hip_check(hipMemcpyAsync(out, d_out, (size_t)b->count * 4 * sizeof(double), hipMemcpyDeviceToHost, s),
+                            "energy download");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "energy sync");
+    if (rc) return rc;
+    const int radial = (b->dim == 1 && b->bc == NLS_BC_REF) ? 1 : 0;
+    for (int i = 0; i < b->count; ++i) {
+        const double dx = b->h_dx[i];
+        const double dV = radial ? 2.0 * M_PI * dx * dx : (b->dim == 1 ? dx : dx * dx);
+        for (int c = 1; c < 4; ++c) out[4 * i + c] *= dV;
+        out[4 * i + 0] = out[4 * i + 1] + out[4 * i + 2] + out[4 * i + 3];
     }
     return NLS_OK;
 }

@@ -36,7 +36,7 @@ EXPORTS = (
     "nls_release", "nls_fill_profile", "nls_fill_state", "nls_slab_halo_rows", "nls_slab_step",
     "nls_make_banded_matrix", "nls_make_laplacian_2d", "nls_rgbmv", "nls_rbbmv", "nls_revervoir",
     "nls_revervoir_2d", "nls_hamiltonian", "nls_hamiltonian_2d", "nls_runge_kutta", "nls_runge_kutta_2d",
-    "nls_batch_set_flags", "nls_batch_observables", "nls_batch_prepare",
+    "nls_batch_set_flags", "nls_batch_observables", "nls_batch_prepare", "nls_batch_energy",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2, "hybrid": 3}
 FLAG_EAGER = 1
@@ -165,6 +165,7 @@ def load():
         "nls_batch_advance": ([vp, i, vp, vp, vp, vp, sz, vp], i),
         "nls_batch_set_flags": ([vp, i], i),
         "nls_batch_observables": ([vp, vp, vp, vp, vp], i),
+        "nls_batch_energy": ([vp, vp, vp, vp, vp, vp], i),
         "nls_batch_prepare": ([vp, i, vp, vp, vp, vp, sz], i),
         "nls_tune_stage_tile": ([i, i], i),
     }

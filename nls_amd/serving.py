@@ -192,7 +192,8 @@ class Group(object):
 
     def retire(self, now_fn):
         """Download and free the members that have finished, with their FP64
-        observables (norm, population, peak density, non-finite count)."""
+        observables (norm, population, peak density, non-finite count,
+        energy)."""
         import ctypes as C
         done = [r for r in self.members if r.remaining == 0]
         if not done:
@@ -201,11 +202,15 @@ class Group(object):
         native._check(native.load().nls_batch_observables(
             self.handle, C.c_void_p(self.psi.data_ptr()), C.c_void_p(self.nres.data_ptr()),
             obs.ctypes.data_as(C.c_void_p), C.c_void_p(self.stream.cuda_stream)))
+        en = np.zeros((len(self.members), 4), np.float64)
+        native._check(native.load().nls_batch_energy(
+            self.handle, C.c_void_p(self.P.data_ptr()), C.c_void_p(self.psi.data_ptr()),
+            C.c_void_p(self.nres.data_ptr()), en.ctypes.data_as(C.c_void_p), C.c_void_p(self.stream.cuda_stream)))
         index = {id(r): i for i, r in enumerate(self.members)}
         for r in done:
             o = obs[index[id(r)]]
             r.observables = {"norm": float(o[0]), "population": float(o[1]), "peak": float(o[2]),
-                             "nonfinite": int(o[3])}
+                             "nonfinite": int(o[3]), "energy": float(en[index[id(r)]][0])}
         for r in done:
             a, b = r.offset, r.offset + r.points
             shape = (r.n,) if r.dim == 1 else (r.n, r.n)

@@ -135,3 +135,28 @@ def test_single_precision_state_accumulates_in_double(gpu):
                         dev(n0.astype(np.float32), "f64"))
     for k in ("kinetic", "interaction", "reservoir"):
         assert abs(e32[k] - e64[k]) <= 1e-5 * abs(e64[k]), (k, e32[k], e64[k])
+
+
+@pytest.mark.parametrize("dim,bc,engine,dtype,dyn", [
+    (2, "periodic", "fused", "f64", True), (2, "ref", "stage", "f64", False), (2, "periodic", "hybrid", "f64", True),
+    (1, "ref", "resident", "f64", True), (1, "periodic", "stage", "f64", False), (2, "periodic", "fused", "f32", True),
+])
+def test_batch_energy_equals_each_member_alone(gpu, dim, bc, engine, dtype, dyn):
+    """nls_batch_energy (a workgroup per member, the member's own stencil
+    weights and radial table) against nls_energy of each member alone."""
+    from nls_amd.batch import Batch
+    sizes = [40, 96, 64, 33] if dim == 2 else [200, 96, 1000]
+    ms = []
+    for i, n in enumerate(sizes):
+        h, psi, P, n0 = smooth_case(dim, n, bc, dyn, seed=20 + i)
+        ms.append(configs.Instance("e%d" % i, dim, n, h, 1e-3, 0, P, psi, n0 if dyn else None, coeffs(dyn)))
+    b = Batch(ms, dim=dim, order=5, bc=bc, dtype=dtype, engine=engine)
+    got = b.energy()
+    tol = 1e-12 if dtype == "f64" else 1e-5
+    for i, m in enumerate(ms):
+        d = native.ProblemDesc.make(dim, m.n, 5, m.dt, m.dx, m.coeffs, bc=bc, dtype=dtype)
+        want = native.energy(d, dev(m.P, dtype), dev(m.psi0, dtype), dev(m.n0, dtype))
+        for c, k in enumerate(("energy", "kinetic", "interaction", "reservoir")):
+            assert abs(got[i, c] - want[k]) <= tol * (abs(want["kinetic"]) + abs(want["interaction"])
+                                                     + abs(want["reservoir"])), (i, k, got[i, c], want[k])
+    b.close()

@@ -78,5 +78,6 @@ def test_result_alone_equals_result_in_the_mix(gpu, oracle):
         dV = r.dx if r.dim == 1 else r.dx * r.dx
         s2 = np.abs(r.psi.astype(np.complex128)) ** 2
         assert r.observables["nonfinite"] == 0
+        assert np.isfinite(r.observables["energy"])
         assert r.observables["norm"] == pytest.approx(s2.sum() * dV, rel=1e-12)
         assert r.observables["population"] == pytest.approx(r.nres.astype(np.float64).sum() * dV, rel=1e-12)
