@@ -395,11 +395,14 @@ def main():
         out["engine_tuning_us_per_step"] = tuning
     # FP64 observables of every member after the timed run (one launch)
     obs = batch.observables()
+    en = batch.energy()
     out["fp64_observables"] = {"members": int(obs.shape[0]), "nonfinite_members": int((obs[:, 3] > 0).sum()),
                                "norm_min": float(obs[:, 0].min()), "norm_max": float(obs[:, 0].max()),
-                               "population_total": float(obs[:, 1].sum())}
-    log("FP64 observables: %d members, %d with non-finite points, norm %.4g .. %.4g" % (
-        obs.shape[0], out["fp64_observables"]["nonfinite_members"], obs[:, 0].min(), obs[:, 0].max()))
+                               "population_total": float(obs[:, 1].sum()),
+                               "energy_min": float(en[:, 0].min()), "energy_max": float(en[:, 0].max())}
+    log("FP64 observables: %d members, %d with non-finite points, norm %.4g .. %.4g, energy %.4g .. %.4g" % (
+        obs.shape[0], out["fp64_observables"]["nonfinite_members"], obs[:, 0].min(), obs[:, 0].max(),
+        en[:, 0].min(), en[:, 0].max()))
 
     if not args.no_plain:
         # plain FP64 path, one instance at a time, on one instance of each size
