@@ -1024,6 +1024,30 @@ static void set_member_steps(nls_batch *b, int i, int steps) {
     ((MemberDev<T> *)b->h_mem.data())[i].steps = steps;
 }
 
+// Copies the points of the batch's members from src to dst, a workgroup per
+// entry of the tile list (tiles of tx x ty points): a call whose state ends in
+// the workspace moves it back into the caller's buffers without touching the
+// gaps between members, which in a hybrid batch hold the resident part's
+// members (stepped concurrently on another stream).
+template <typename T>
+__global__ void __launch_bounds__(256) k_batch_carry(const MemberDev<T> *mem, const TileDesc *tiles, int ntiles,
+                                                     int dim, int tx, int ty, const cplx_t<T> *src, const T *srcn,
+                                                     cplx_t<T> *dst, T *dstn) {
+    const int t = blockIdx.x;
+    if (t >= ntiles) return;
+    const TileDesc td = tiles[t];
+    const MemberDev<T> m = mem[td.m];
+    const int n = m.n;
+    const long long o = m.off;
+    for (int k = threadIdx.x; k < tx * ty; k += 256) {
+        const int gx = td.x0 + k % tx, gy = td.y0 + k / tx;
+        if (gx >= n || gy >= (dim == 2 ? n : 1)) continue;
+        const size_t gi = (size_t)o + (dim == 2 ? (size_t)gy * n + gx : (size_t)gx);
+        dst[gi] = src[gi];
+        if (dstn) dstn[gi] = srcn[gi];
+    }
+}
+
 // Steps it0 .. it0 + iters - 1 of the fused or per-stage engine, each
 // launch issued on s.
 template <typename T>
@@ -1083,9 +1107,14 @@ static int batch_steps(nls_batch *b, int it0, int iters, const T *P, cplx_t<T> *
     int rc = hip_check(hipGetLastError(), "batch launch");
     if (rc) return rc;
     if (cur != psi) {
-        rc = hip_check(hipMemcpyAsync(psi, cur, b->points * sizeof(C), hipMemcpyDeviceToDevice, s), "copy state");
-        if (!rc && dyn)
-            rc = hip_check(hipMemcpyAsync(nres, curn, b->points * sizeof(T), hipMemcpyDeviceToDevice, s), "copy reservoir");
+        // an odd step count ends in the workspace: carry the members' points
+        // (only theirs) back
+        const int tx = b->dim == 1 ? TILE1 : 64;
+        const int ty = b->dim == 1 ? 1 : b->engine == NLS_ENGINE_FUSED ? fused_ty(b->order, b->fform) : BATCH_STAGE_TY;
+        hipLaunchKernelGGL((k_batch_carry<T>), dim3(b->ntiles), dim3(256), 0, s, (const MemberDev<T> *)b->d_mem,
+                           (const TileDesc *)b->d_tiles, b->ntiles, b->dim, tx, ty, (const C *)cur,
+                           dyn ? (const T *)curn : nullptr, psi, dyn ? nres : nullptr);
+        rc = hip_check(hipGetLastError(), "carry launch");
     }
     return rc;
 }

@@ -322,3 +322,38 @@ def test_resident_budgets_and_invariance(gpu, oracle):
                          small[2].n0)
     assert rel(b.member(2)[0], opsi) <= TOL32
     b.close()
+
+
+@pytest.mark.parametrize("dtype", ("f32", "f64"))
+@pytest.mark.parametrize("dim", (1, 2))
+@pytest.mark.parametrize("steps", (61, 129))
+def test_hybrid_odd_step_counts(gpu, dtype, dim, steps):
+    """An advance of an odd number of steps (61: direct launches; 129: two
+    64-step graph replays and one direct step) leaves the fused or per-stage
+    part's state in the workspace.  Only that part's own points may go back:
+    the resident members sit between them in the packed arrays and are
+    stepped concurrently on the batch's side stream.  Every member equals the
+    same member alone on its engine, bit for bit, after one such call and
+    after a second one."""
+    if dim == 2:
+        ms = members_2d([64, 128, 40, 96, 64, 33, 200], seed=17, spacing=0.15)
+        rest, fits = "fused", (lambda m: m.n <= 64)
+    else:
+        ms = members_1d([300, 5000, 4096, 6000, 1000], seed=31)
+        rest, fits = "stage", (lambda m: m.n <= 4096)
+    assert any(fits(m) for m in ms) and not all(fits(m) for m in ms)
+    mix = Batch(ms, dim=dim, order=5, bc="periodic", dtype=dtype, engine="hybrid")
+    alone = [Batch([m], dim=dim, order=5, bc="periodic", dtype=dtype, engine="resident" if fits(m) else rest)
+             for m in ms]
+    for call in range(2):
+        mix.advance(steps)
+        for i, a in enumerate(alone):
+            a.advance(steps)
+            psi, nres = mix.member(i)
+            apsi, anres = a.member(0)
+            assert np.all(np.isfinite(apsi)), (call, i)
+            np.testing.assert_array_equal(apsi, psi, err_msg="call %d member %d" % (call, i))
+            np.testing.assert_array_equal(anres, nres, err_msg="call %d member %d" % (call, i))
+    for a in alone:
+        a.close()
+    mix.close()
