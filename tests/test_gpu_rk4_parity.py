@@ -194,3 +194,63 @@ def test_dropin_chemical_potential(gpu, oracle, dim):
         mu = native.nls.chemical_potential_2d(dx, P, c, u0)
         ref = oracle.chemical_potential(2, dx, n, P, c, u0)
     assert abs(mu - ref) <= 1e-4 * max(1.0, abs(ref))
+
+
+_SQ2048 = {}
+
+
+def _square2048_oracle(oracle, steps=10):
+    """The oracle's first `steps` steps of the 2048^2 workload (computed once)."""
+    import os
+    if steps not in _SQ2048:
+        inst = configs.square2048()
+        oracle.set_threads(min(16, os.cpu_count() or 1))
+        opsi, onres = oracle.rk4(2, inst.n, 5, 1, inst.dt, inst.dx, steps, inst.P, inst.coeffs, inst.psi0, inst.n0)
+        _SQ2048[steps] = (inst, opsi, onres)
+    inst, opsi, onres = _SQ2048[steps]
+    return inst, steps, opsi, onres
+
+
+@pytest.mark.parametrize("engine", ("fused", "stage"))
+def test_square2048_f64_first_steps(gpu, oracle, engine):
+    """The 2048^2 workload itself (periodic on [-64,64)^2, ring pumping P0 = 3,
+    r0 = 20, w = 2, dt = 2.5e-4, dynamic reservoir, order 5) for its first 10
+    steps on each 2-D RK4 engine of the plain path: the fused kernel in the
+    form this launch takes on MI355X (512-thread 64 x 16 tiles, two
+    workgroups a CU: 4096 64 x 32 tiles are more than two rounds of 256 CUs)
+    and the per-stage kernels, against the oracle in double precision
+    (<= 1e-10); single precision on the same engine within 1e-4 of it."""
+    inst, steps, opsi, onres = _square2048_oracle(oracle)
+    desc = native.ProblemDesc.make(2, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic", dtype="f64",
+                                   engine=engine)
+    psi, nres = native.solve_host(desc, steps, inst.P, inst.psi0, inst.n0)
+    assert np.all(np.isfinite(psi))
+    assert rel(psi, opsi) <= TOL64, rel(psi, opsi)
+    assert rel(nres, onres) <= TOL64, rel(nres, onres)
+    desc32 = native.ProblemDesc.make(2, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic", dtype="f32",
+                                     engine=engine)
+    psi32, nres32 = native.solve_host(desc32, steps, inst.P, inst.psi0, inst.n0)
+    assert rel(psi32, opsi) <= TOL32, rel(psi32, opsi)
+    assert rel(nres32, onres) <= TOL32
+
+
+@pytest.mark.parametrize("dim", (1, 2))
+def test_unstable_dt_is_reported(gpu, dim):
+    """A dt far above the RK4 stability limit (|lambda| dt <= 2 sqrt 2) blows
+    the solution up.  The extended host entry point reports it
+    (NLS_ENONFINITE -> FloatingPointError); the drop-in keeps the reference's
+    behaviour (nls.f90:725-733): it returns the non-finite state silently."""
+    n = 64 if dim == 1 else 32
+    P, psi0, n0, c = small_case(dim, n, "periodic", True)
+    dx = 0.1
+    lam = dim * (16.0 / 3.0) / dx ** 2                 # largest |eigenvalue| of the order-5 Laplacian
+    dt = 20.0 * 2.0 * np.sqrt(2.0) / lam
+    desc = native.ProblemDesc.make(dim, n, 5, dt, dx, c, bc="periodic", dtype="f64")
+    with pytest.raises(FloatingPointError):
+        native.solve_host(desc, 400, P, psi0, n0)
+    stable = native.ProblemDesc.make(dim, n, 5, 0.1 * dt / 20.0, dx, c, bc="periodic", dtype="f64")
+    psi, _ = native.solve_host(stable, 50, P, psi0, n0)
+    assert np.all(np.isfinite(psi))
+    fn = native.nls.solve_nls if dim == 1 else native.nls.solve_nls_2d
+    u = fn(dt, dx, 5, 400, P, configs.model_coeffs(dynamic=False), psi0)
+    assert not np.all(np.isfinite(u))
