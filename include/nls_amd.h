@@ -166,7 +166,11 @@ typedef struct nls_problem {
 
 /* Host-memory solve with the reference's semantics and the extensions:
  * advances psi (complex, dtype) and, when coeffs[10] != 0, the reservoir n
- * (real, dtype) in place by iters steps.  pumping is real (dtype). */
+ * (real, dtype) in place by iters steps.  pumping is real (dtype).
+ * Returns NLS_ENONFINITE when the final state holds NaN or Inf (a dt above
+ * the scheme's stability limit); the state is written back all the same.
+ * The reference reports nothing in that case (nls.f90:725-733), and neither
+ * do the drop-ins above (nls_solve_nls*), which keep its behaviour. */
 int nls_solve_host(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres);
 
 /* Workspace needed by nls_advance for p (bytes). */

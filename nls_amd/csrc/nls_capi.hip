@@ -4,6 +4,7 @@
 // procedures.
 #include <cstdarg>
 #include <cstdio>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -49,6 +50,7 @@ int check_problem(const nls_problem *p) {
         return fail(NLS_EINVAL, "engine must be one of NLS_ENGINE_* (got %d)", p->engine);
     if (p->engine == NLS_ENGINE_FUSED && p->dim != 2)
         return fail(NLS_EUNSUPPORTED, "the fused engine is 2-D only");
+
     if (p->engine == NLS_ENGINE_RESIDENT && p->scheme == NLS_SCHEME_RK4 && !resident_fits(p))
         return fail(NLS_EUNSUPPORTED, "the resident engine holds 1-D n <= 4096 or 2-D n <= 64 "
                                       "(not double-precision 2-D order 7)");
@@ -408,7 +410,8 @@ int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const vo
     return hip_check(hipStreamSynchronize(s), "RHS sync");
 }
 
-int nls_solve_host(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres) {
+// Host-memory solve; `check` scans the downloaded state for NaN/Inf.
+static int solve_host(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, bool check) {
     int rc = check_problem(p);
     if (rc) return rc;
     if (iters < 0) return fail(NLS_EINVAL, "iters must be >= 0");
@@ -431,8 +434,28 @@ int nls_solve_host(const nls_problem *p, int iters, const void *pumping, void *p
     if ((rc = hip_check(hipDeviceSynchronize(), "solve")) ||
         (rc = hip_check(hipMemcpy(psi, dpsi.ptr, N * 2 * rs, hipMemcpyDeviceToHost), "download psi")))
         return rc;
-    if (dyn) rc = hip_check(hipMemcpy(nres, dn.ptr, N * rs, hipMemcpyDeviceToHost), "download reservoir");
-    return rc;
+    if (dyn && (rc = hip_check(hipMemcpy(nres, dn.ptr, N * rs, hipMemcpyDeviceToHost), "download reservoir")))
+        return rc;
+    if (check) {
+        size_t bad = 0;
+        if (p->dtype == NLS_F64) {
+            const double *a = (const double *)psi, *b = (const double *)nres;
+            for (size_t i = 0; i < 2 * N; ++i) bad += !std::isfinite(a[i]);
+            for (size_t i = 0; dyn && i < N; ++i) bad += !std::isfinite(b[i]);
+        } else {
+            const float *a = (const float *)psi, *b = (const float *)nres;
+            for (size_t i = 0; i < 2 * N; ++i) bad += !std::isfinite(a[i]);
+            for (size_t i = 0; dyn && i < N; ++i) bad += !std::isfinite(b[i]);
+        }
+        if (bad)
+            return fail(NLS_ENONFINITE, "the solution holds %zu non-finite values after %d steps (dt = %g above the "
+                        "scheme's stability limit?); the state was written back", bad, iters, p->dt);
+    }
+    return NLS_OK;
+}
+
+int nls_solve_host(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres) {
+    return solve_host(p, iters, pumping, psi, nres, true);
 }
 
 // ---- drop-ins of the reference's Fortran procedures ----------------------
@@ -451,7 +474,9 @@ static int dropin_solve(int dim, float dt, float dx, int n, int order, int iters
     if (rc) return rc;
     const size_t N = points_of(&p);
     if (u != u0) memcpy(u, u0, N * 2 * sizeof(float));
-    return nls_solve_host(&p, iters, pumping, u, nullptr);
+    // the reference reports nothing when a solve blows up (nls.f90:725-733):
+    // the drop-ins return the state as it is
+    return solve_host(&p, iters, pumping, u, nullptr, false);
 }
 
 int nls_solve_nls(float dt, float dx, int n, int order, int iters, const float *pumping, const float *coeffs,
