@@ -47,6 +47,25 @@ def version():
     return 0, 2, 0
 
 
+# The .mat "model" field as the reference writes it, str(type(self)) of its
+# classes in package nls (nls/model.py:265), and the strings its Problem.model
+# dispatch accepts (:67-72): files written here restore in the reference, and
+# the reference's files restore here.
+REF_MODEL_NAMES = {1: "<class 'nls.model.Model1D'>", 2: "<class 'nls.model.Model2D'>"}
+
+
+def _model_kind(name):
+    """1 or 2 for a model name Problem.model accepts, else None: '1d',
+    'default', '2d', str() of this module's classes or of the reference's,
+    and the bare class names."""
+    name = str(name).strip()
+    if name in ("1d", "default", str(Model1D), REF_MODEL_NAMES[1], "Model1D"):
+        return 1
+    if name in ("2d", str(Model2D), REF_MODEL_NAMES[2], "Model2D"):
+        return 2
+    return None
+
+
 class Problem(object):
     """Factory of models (model.py:27-117)."""
 
@@ -59,20 +78,20 @@ class Problem(object):
         for k, v in DEFAULT_ORIGINALS.items():
             originals.setdefault(k, v)
         kwargs["original_params"] = originals
-        if kwargs["model"] in ("1d", "default", str(Model1D)):
+        kind = _model_kind(kwargs["model"])
+        if kind == 1:
             return self.fabricateModel1D(*args, **kwargs)
-        if kwargs["model"] in ("2d", str(Model2D)):
+        if kind == 2:
             return self.fabricateModel2D(*args, **kwargs)
         raise Exception("Unknown model passed!")
 
     def modelFromFile(self, filename):
+        """model.py:74-84: a default model of the file's kind, then restore."""
         from scipy.io import loadmat
         mat = loadmat(filename)
         if "model" not in mat:
             raise ValueError("%s holds no model" % filename)
-        kind = str(mat["model"][0])
-        model = self.model(model="2d" if "2D" in kind or "2d" in kind else "1d")
-        return model.restore(filename)
+        return self.model(model=str(mat["model"][0])).restore(filename)
 
     @staticmethod
     def _grid(kwargs, n):
@@ -249,7 +268,7 @@ class AbstractModel(object):
         date = date if date else datetime.now()
         date = date.replace(microsecond=0).isoformat()
         filename = filename if filename else date + ".mat"
-        mat = {"model": str(type(self).__name__), "date": date, "dim": self.init_sol.ndim,
+        mat = {"model": REF_MODEL_NAMES[self.init_sol.ndim], "date": date, "dim": self.init_sol.ndim,
                "dimlesses": self.coeffs, "init_solution": self.init_sol, "num_iters": self.num_iters,
                "num_nodes": self.num_nodes, "order": self.order, "originals": self.originals,
                "pumping": self.getPumping(), "spatial_step": self.dx, "time_step": self.dt,

@@ -82,3 +82,48 @@ def test_model_checkpoint_through_mat_file(gpu, tmp_path):
     second = resumed.solve()
     np.testing.assert_array_equal(second.getSolution(), full.getSolution())
     np.testing.assert_array_equal(second.getReservoir(), full.getReservoir())
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b)))
+
+
+def test_resume_reference_checkpoint_matches_reference(gpu):
+    """A checkpoint the reference wrote (its .mat layout, its own FP32
+    solution after N steps; tests/golden/make_mat_fixture.py) resumed the
+    reference's way (setInitialSolution, then solve: tools/check.py:34-36)
+    for N more steps equals the reference's uninterrupted 2N-step solve to
+    FP32 rounding (relative L2 <= 1e-5)."""
+    import os
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    m = M.Problem().model(filename=os.path.join(gold, "ref_layout_model2d.mat"))
+    sol = M.Solution(m).restore(os.path.join(gold, "ref_layout_model2d.mat"))
+    ref = np.load(os.path.join(gold, "ref_layout_model2d.npz"))
+    m.setInitialSolution(sol.getSolution())
+    resumed = m.solve().getSolution()
+    assert resumed.dtype == np.complex64
+    assert _rel(resumed, ref["u_2n"]) <= 1e-5, _rel(resumed, ref["u_2n"])
+
+
+@pytest.mark.parametrize("dim,n", [(1, 300), (2, 48)])
+def test_mat_resume_matches_oracle_uninterrupted(gpu, oracle, tmp_path, dim, n):
+    """Double precision, dynamic reservoir: N steps, a checkpoint of psi and
+    the reservoir through the model's .mat file, N more steps -- against the
+    oracle's uninterrupted 2N-step solve (relative L2 <= 1e-10)."""
+    N = 140
+    h, P, psi0, n0, c = case(dim, n)
+    dt = 5e-4 if dim == 1 else 1e-3
+    pump = PU.GridPumping(P)
+
+    def model(u0, r0, iters):
+        return M.Problem().model(model="%dd" % dim, dx=h, dt=dt, order=5, num_nodes=n, num_iters=iters, u0=u0,
+                                 pumping=pump, coeffs=c, bc="periodic", dtype="f64", n0=r0)
+
+    first = model(psi0, n0, N).solve()
+    ck = model(first.getSolution(), first.getReservoir(), N)
+    path = str(tmp_path / "ck.mat")
+    ck.store(path)
+    second = M.Problem().model(filename=path).solve()
+    opsi, onres = oracle.rk4(dim, n, 5, 1, dt, h, 2 * N, P, c, psi0, n0)
+    assert _rel(second.getSolution(), opsi) <= 1e-10, _rel(second.getSolution(), opsi)
+    assert _rel(second.getReservoir(), onres) <= 1e-10
