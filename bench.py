@@ -29,14 +29,21 @@ sys.path.insert(0, ROOT)
 METRIC = "grid-point-steps/s batched FP32 vs plain FP64 (relL2≤1e-4); p50/p99 latency"
 HBM_PEAK = 8.0e12          # bytes/s, MI355X
 FUSED_BYTES_F32_DYN = 28   # algorithmic bytes per grid-point-step of the fused step kernel: 2 S + r
+# Vector ALU issue rate of MI355X without packed FP32: 256 CUs x 4 SIMDs, a
+# wave64 instruction occupies a 16-lane SIMD for 4 cycles, 2.4 GHz
+VALU_LANES_PEAK = 256 * 4 * 16 * 2.4e9      # lane-instructions/s
+SWEEP_STEPS = 5000         # the workload's step count (accuracy and the plain FP64 leg use it whatever --steps is)
+GRAPH_MIN_STEPS = 128      # a batch call replays 64-step graphs from this many steps on (include/nls_amd.h)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def read_traffic(name):
-    """HBM bytes per launch measured by rocprofv3 --pmc (profiles/<name>), or None."""
+def read_profile_value(name, key):
+    """A `key = value` figure of a profile summary (profiles/<name>): HBM bytes
+    per launch from rocprofv3 --pmc FETCH_SIZE/WRITE_SIZE, VALU lane-instructions
+    per grid-point-step from SQ_INSTS_VALU; None when absent."""
     path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
@@ -46,20 +53,38 @@ def read_traffic(name):
             k, v = line.split("=", 1)
             vals[k.strip()] = v.strip()
     try:
-        return float(vals["traffic_bytes_per_launch"])
+        return float(vals[key])
     except (KeyError, ValueError):
         return None
 
 
+def host_cores():
+    """Host cores this process may use (the box's CPU share caps at 16)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
 def cpu_baseline(members, steps):
-    """Time the reference Fortran (oracle/_ref, FP32, one core) -- or, where it
-    is not built, the plain-C oracle -- on a bounded sample of the workload."""
+    """The CPU path timed on this host, on bounded samples:
+
+    * headline: the reference Fortran (oracle/_ref, FP32, BLAS sgbmv, one core)
+      running solve_nls_2d on sweep512 members -- or, where it is not built,
+      the C restatement on one core;
+    * line256_reference: the reference's solve_nls on the 1-D N=256 workload in
+      full (2000 steps), one core;
+    * oracle_all_cores: the C restatement (FP32, the workload's own periodic
+      grid and reservoir equation) on every host core, same members."""
     import ctypes
     import numpy as np
     os.environ["MKL_NUM_THREADS"] = "1"
+    from nls_amd import configs
     from oracle import oracle as O
     sample = "sweep512 members 0-%d, %d RK4 steps each" % (len(members) - 1, steps)
     work = sum(m.points for m in members) * steps
+    out = None
     try:
         R = O.RefLib(32)
         try:
@@ -70,18 +95,36 @@ def cpu_baseline(members, steps):
         for m in members:
             R.solve(2, m.dt, m.dx, m.n, 5, steps, m.P, m.coeffs, m.psi0)
         t = time.perf_counter() - t0
-        return {"value": work / t, "unit": "grid-point-steps/s", "cores": 1, "kind": "reference",
-                "sample": sample + "; reference Fortran solve_nls_2d (FP32, BLAS sgbmv, MKL_NUM_THREADS=1) "
-                          "with its own boundaries and algebraic reservoir (it has no periodic grid or "
-                          "reservoir equation)"}
+        out = {"value": work / t, "unit": "grid-point-steps/s", "cores": 1, "kind": "reference",
+               "sample": sample + "; reference Fortran solve_nls_2d (FP32, BLAS sgbmv, MKL_NUM_THREADS=1) "
+                         "with its own boundaries and algebraic reservoir (it has no periodic grid or "
+                         "reservoir equation)"}
+        line = configs.line256()
+        t0 = time.perf_counter()
+        R.solve(1, line.dt, line.dx, line.n, 5, line.steps, line.P, line.coeffs, line.psi0)
+        t = time.perf_counter() - t0
+        out["line256_reference"] = {
+            "value": line.points * line.steps / t, "unit": "grid-point-steps/s", "cores": 1, "seconds": t,
+            "sample": "solve_nls, 1-D N=256, %d steps of %g, order 5, in full; the reference's radial grid "
+                      "and algebraic reservoir" % (line.steps, line.dt)}
     except (OSError, FileNotFoundError):
         O.set_threads(1)
         t0 = time.perf_counter()
         for m in members:
             O.rk4(2, m.n, 5, 1, m.dt, m.dx, steps, m.P, m.coeffs, m.psi0, m.n0, dtype=np.float32)
         t = time.perf_counter() - t0
-        return {"value": work / t, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
-                "sample": sample + "; C restatement, FP32, same boundaries and reservoir"}
+        out = {"value": work / t, "unit": "grid-point-steps/s", "cores": 1, "kind": "port",
+               "sample": sample + "; C restatement, FP32, same boundaries and reservoir"}
+    cores = host_cores()
+    O.set_threads(cores)
+    t0 = time.perf_counter()
+    for m in members:
+        O.rk4(2, m.n, 5, 1, m.dt, m.dx, steps, m.P, m.coeffs, m.psi0, m.n0, dtype=np.float32)
+    t = time.perf_counter() - t0
+    out["oracle_all_cores"] = {"value": work / t, "unit": "grid-point-steps/s", "cores": cores, "kind": "port",
+                               "sample": sample + "; C restatement, FP32, periodic, reservoir equation, "
+                                                  "%d threads" % cores}
+    return out
 
 
 def single_grid_2048(dev, steps=100):
@@ -247,6 +290,87 @@ def config_accuracy(dev):
     return out
 
 
+def sweep512_accuracy(members, engine, dev, steps=SWEEP_STEPS):
+    """Every sweep512 member after the workload's full step count: the
+    batched complex64 solve on `engine` against the batched complex128 solve
+    of the same inputs (AUTO engine), relative L2 per member.  The FP64 batch
+    stands for the plain FP64 path: the tests hold every batch member bitwise
+    equal to the member solved alone and within 1e-10 of the oracle."""
+    import numpy as np
+    import torch
+    from nls_amd.batch import Batch
+    states, secs = {}, {}
+    offs = sizes = None
+    for dtype, eng in (("f32", engine), ("f64", "auto")):
+        b = Batch(members, dim=2, order=5, bc="periodic", dtype=dtype, engine=eng, device=dev)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b.advance(steps)
+        torch.cuda.synchronize()
+        secs[dtype] = time.perf_counter() - t0
+        states[dtype] = b.psi.cpu().numpy().astype(np.complex128)
+        offs, sizes = b.offsets, b.sizes
+        b.close()
+    errs = np.empty(len(members))
+    for i in range(len(members)):
+        a, e = offs[i], offs[i] + sizes[i]
+        ref = states["f64"][a:e]
+        errs[i] = np.linalg.norm(states["f32"][a:e] - ref) / np.linalg.norm(ref)
+    per_size = {}
+    for n in sorted({m.n for m in members}):
+        sel = [i for i, m in enumerate(members) if m.n == n]
+        per_size[str(n)] = {"members": len(sel), "max_relL2": float(errs[sel].max()),
+                            "median_relL2": float(np.median(errs[sel]))}
+    out = {"steps": steps, "members": len(members), "max_relL2_vs_f64": float(errs.max()), "tolerance": 1e-4,
+           "within_tolerance": bool(errs.max() <= 1e-4), "per_size": per_size,
+           "f32_batch_s": secs["f32"], "f64_batch_s": secs["f64"],
+           "reference": "complex128 batch of the same inputs, %d steps" % steps}
+    log("accuracy sweep512 (%d steps, all %d members): max FP32 vs FP64 relL2 = %.2e (%s)" % (
+        steps, len(members), errs.max(), ", ".join("%s^2 %.1e" % (k, v["max_relL2"]) for k, v in per_size.items())))
+    return out
+
+
+def plain_f64(members, dev, steps=SWEEP_STEPS):
+    """The plain FP64 path (nls_advance, one instance at a time, AUTO engine)
+    on one instance of each sweep512 size: warmed up (kernel modules loaded,
+    the 64-step graph captured), state reset, then the workload's full step
+    count timed; the rate over the workload's size mix weights each size by
+    its members."""
+    import numpy as np
+    import torch
+    from nls_amd import native
+    sample = {}
+    for i, m in enumerate(members):
+        sample.setdefault(m.n, i)
+    rates = {}
+    for n, i in sorted(sample.items()):
+        m = members[i]
+        d = native.ProblemDesc.make(2, n, 5, m.dt, m.dx, m.coeffs, bc="periodic", dtype="f64")
+        P = torch.from_numpy(np.ascontiguousarray(m.P)).to(dev)
+        psi0 = torch.from_numpy(np.ascontiguousarray(m.psi0, np.complex128)).to(dev)
+        n0 = torch.from_numpy(np.ascontiguousarray(m.n0)).to(dev)
+        psi, nres = psi0.clone(), n0.clone()
+        ws = torch.empty(max(1, native.workspace_bytes(d)), dtype=torch.uint8, device=dev)
+        native.advance(d, GRAPH_MIN_STEPS, P, psi, nres, ws)       # warm-up on the timed buffers
+        psi.copy_(psi0)
+        nres.copy_(n0)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        native.advance(d, steps, P, psi, nres, ws)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t1
+        rates[n] = n * n * steps / t
+        log("plain FP64 %d^2: %.3f G grid-point-steps/s (%d steps, %.1f us/step)" % (n, rates[n] / 1e9, steps,
+                                                                                    t / steps * 1e6))
+    points = sum(m.points for m in members)
+    t_plain = sum(m.points * steps / rates[m.n] for m in members)
+    return {"value": points * steps / t_plain, "unit": "grid-point-steps/s", "steps": steps,
+            "per_size": {str(k): v for k, v in rates.items()},
+            "note": "plain RK4 FP64 path (nls_advance, AUTO engine), one instance at a time; rate per size on one "
+                    "instance of each size over the workload's %d steps after a warm-up, weighted by the size "
+                    "mix" % steps}
+
+
 def serving(rate, window, seed=0):
     """Continuous batching: Poisson arrivals at `rate` per second over `window`
     seconds, the request mix of nls_amd.serving.make_request with inputs given
@@ -358,6 +482,7 @@ def main():
     launch_s = gpu_s / K             # one fused-kernel launch per step; events on its stream
     log("batched FP32: %.3fs wall, %.1f us/step, %.2f G grid-point-steps/s" % (wall, wall / K * 1e6, value / 1e9))
 
+    graphs = K >= GRAPH_MIN_STEPS
     if engine in ("fused", "hybrid"):
         # hybrid: the 64^2 members run on the resident engine on a side stream
         # (state on chip, HBM at the ends of a call); the fused kernel, the
@@ -365,12 +490,22 @@ def main():
         fused_points = points if engine == "fused" else sum(m.points for m in members if m.n * m.n > 4096)
         bytes_per_launch = fused_points * FUSED_BYTES_F32_DYN
         achieved = bytes_per_launch / launch_s
-        traffic = read_traffic("fused_step_f32_sweep512_traffic.txt" if engine == "fused" else
-                               "fused_step_f32_sweep512_hybrid_traffic.txt")
-        roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+        prof = ("fused_step_f32_sweep512_traffic.txt" if engine == "fused" else
+                "fused_step_f32_sweep512_hybrid_traffic.txt")
+        traffic = read_profile_value(prof, "traffic_bytes_per_launch")
+        valu_per_point = read_profile_value(prof, "valu_lane_insts_per_point_step")
+        roofline = {"bound": "valu", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                     "kernel": "k_fused_step_2d<float,5,periodic,dynamic>", "kernel_points": fused_points,
-                    "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6}
+                    "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6,
+                    "note": "achieved/peak/frac: the HBM roofline of the algorithmic bytes (secondary); the "
+                            "kernel's limiter is its vector ALU work (valu)"}
+        if valu_per_point:
+            va = valu_per_point * fused_points / launch_s
+            roofline["valu"] = {"lane_insts_per_point_step": valu_per_point, "achieved": va, "peak": VALU_LANES_PEAK,
+                                "unit": "lane-instructions/s", "frac": va / VALU_LANES_PEAK,
+                                "source": "rocprofv3 SQ_INSTS_VALU x 64 / points per launch (profiles/%s); peak: "
+                                          "1024 SIMDs x 16 lanes x 2.4 GHz, no packed FP32" % prof}
     else:
         # per-stage engine: four launches per step, 15 S + 4 r bytes per point-step
         bytes_per_step = points * (15 * 12 + 4 * 4)
@@ -379,16 +514,21 @@ def main():
                     "frac": achieved / HBM_PEAK, "traffic": None, "kernel": "k_batch_stage (4 per step)",
                     "algorithmic_bytes_per_step": bytes_per_step, "avg_step_us": launch_s * 1e6}
 
+    if engine == "resident":
+        blocks = "one resident launch per call"
+    else:
+        part = "fused part" if engine == "hybrid" else "every step"
+        blocks = ("%s: 64-step hipGraph replays (%d steps >= %d)" % (part, K, GRAPH_MIN_STEPS) if graphs else
+                  "%s: direct launches, no graph replay (%d steps < %d)" % (part, K, GRAPH_MIN_STEPS))
+        if engine == "hybrid":
+            blocks += "; resident part (64^2 members, own stream): one launch per call"
     out = {"metric": METRIC, "value": value, "unit": "grid-point-steps/s", "n_gpus": 1, "steps": K, "warmup": W,
            "ms_per_step": wall / K * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
            "dtype": "f32", "data": "synthetic",
            "config": {"workload": "sweep512: 512 2-D instances, sides 64/128/256 (p=0.5/0.3/0.2), periodic, "
                                   "order 5, dynamic reservoir, dt=5e-4, %d RK4 steps, complex64 state" % K,
                       "instances": len(members), "grid_points": points, "engine": engine,
-                      "step_blocks": {"resident": "one launch per call",
-                                      "hybrid": "fused part: 64-step hipGraph replays; resident part (64^2 members, "
-                                                "own stream): one launch per call"}.get(engine,
-                                                                                       "64-step hipGraph replays"),
+                      "step_blocks": blocks, "graphs_replayed": bool(graphs and engine != "resident"),
                       "parallelism": "single GPU"},
            "roofline": roofline}
     if tuning is not None:
@@ -403,46 +543,28 @@ def main():
     log("FP64 observables: %d members, %d with non-finite points, norm %.4g .. %.4g, energy %.4g .. %.4g" % (
         obs.shape[0], out["fp64_observables"]["nonfinite_members"], obs[:, 0].min(), obs[:, 0].max(),
         en[:, 0].min(), en[:, 0].max()))
+    batch.close()
 
     if not args.no_plain:
-        # plain FP64 path, one instance at a time, on one instance of each size
-        sample = {}
-        for i, m in enumerate(members):
-            if m.n not in sample:
-                sample[m.n] = i
-        rates, errs = {}, {}
-        for n, i in sorted(sample.items()):
-            m = members[i]
-            d = native.ProblemDesc.make(2, n, 5, m.dt, m.dx, m.coeffs, bc="periodic", dtype="f64")
-            P = torch.from_numpy(np.ascontiguousarray(m.P)).to(dev)
-            psi = torch.from_numpy(np.ascontiguousarray(m.psi0, np.complex128)).to(dev)
-            nres = torch.from_numpy(np.ascontiguousarray(m.n0)).to(dev)
-            ws = torch.empty(native.workspace_bytes(d), dtype=torch.uint8, device=dev)
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            native.advance(d, K, P, psi, nres, ws)
-            torch.cuda.synchronize()
-            t = time.perf_counter() - t1
-            rates[n] = n * n * K / t
-            ref = psi.cpu().numpy()
-            got, _ = batch.member(i)
-            errs[n] = float(np.linalg.norm(got - ref) / np.linalg.norm(ref))
-            log("plain FP64 %d^2: %.3f G grid-point-steps/s, batched FP32 relL2 = %.2e" % (n, rates[n] / 1e9, errs[n]))
-        # aggregate plain rate over the workload's size mix
-        t_plain = sum(m.points * K / rates[m.n] for m in members)
-        plain = points * K / t_plain
-        out["plain_f64"] = {"value": plain, "unit": "grid-point-steps/s", "per_size": {str(k): v for k, v in rates.items()},
-                            "note": "plain RK4 FP64 path, one instance at a time, rate per size measured on one "
-                                    "instance of each size over the full step count, weighted by the workload mix"}
-        out["speedup_vs_plain_f64"] = value / plain
-        out["accuracy"] = {"max_relL2_vs_f64": max(errs.values()), "per_size": {str(k): v for k, v in errs.items()},
-                           "tolerance": 1e-4}
+        # the headline's qualifier at the workload's full step count, whatever
+        # --steps is: every member of the batched FP32 solve against the FP64
+        # solve of the same inputs, and the plain FP64 path's rate
+        acc = sweep512_accuracy(members, engine, dev)
+        out["accuracy"] = acc
+        plain = plain_f64(members, dev)
+        out["plain_f64"] = plain
+        out["speedup_vs_plain_f64"] = value / plain["value"]
+        log("batched FP32 / plain FP64 = %.1fx" % out["speedup_vs_plain_f64"])
         out["single_grid_2048"] = single_grid_2048(dev)
         out["slab_shards"] = slab_shards(dev)
 
-    batch.close()
     if not args.no_accuracy:
         out["config_accuracy"] = config_accuracy(dev)
+        if "accuracy" in out:
+            a = out["accuracy"]
+            out["config_accuracy"]["sweep512"] = {"relL2_f32_vs_f64": a["max_relL2_vs_f64"], "steps": a["steps"],
+                                                  "members": a["members"], "per_size": a["per_size"],
+                                                  "note": "largest over all members; see accuracy"}
     if not args.no_serve:
         out["serving"] = [serving(float(r), args.serve_window) for r in args.serve_rates.split(",") if r]
     if not args.no_cpu:

@@ -35,5 +35,12 @@ def test_bench_prints_one_json_line(gpu):
     assert out["value"] > 0 and out["dtype"] == "f32" and out["vs_baseline"] is None
     assert "workload" in out["config"] and out["config"]["engine"] in ("fused", "hybrid", "stage", "resident")
     rl = out["roofline"]
-    assert rl["bound"] == "hbm" and rl["unit"] == "GB/s" and rl["peak"] == 8000.0
+    assert rl["bound"] in ("hbm", "valu") and rl["unit"] == "GB/s" and rl["peak"] == 8000.0
     assert abs(rl["frac"] - rl["achieved"] / rl["peak"]) < 1e-12
+    if "valu" in rl:
+        v = rl["valu"]
+        assert v["unit"] == "lane-instructions/s" and 0 < v["frac"] < 1
+        assert abs(v["achieved"] - v["lane_insts_per_point_step"] * rl["kernel_points"] / (rl["avg_launch_us"] * 1e-6)) \
+            <= 1e-9 * v["achieved"]
+    # 16 steps: below the 128 a call needs before it replays step-block graphs
+    assert out["config"]["graphs_replayed"] is False
