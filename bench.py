@@ -371,24 +371,33 @@ def plain_f64(members, dev, steps=SWEEP_STEPS):
                     "mix" % steps}
 
 
-def serving(rate, window, seed=0):
-    """Continuous batching: Poisson arrivals at `rate` per second over `window`
-    seconds, the request mix of nls_amd.serving.make_request with inputs given
-    by parameters and generated on the device at admission; latency is
-    arrival to result on the host."""
+def serving(rates, capacity_ref=None):
+    """Continuous batching: Poisson arrivals at each rate over a window of
+    about 200 requests, the request mix of nls_amd.serving.make_request with
+    inputs given by parameters and generated on the device at admission;
+    latency is arrival to result on the host.  Beside the percentiles: the
+    steady-state rate (work of the blocks run while requests keep arriving,
+    the drain after the last arrival left out), the offered load, and the
+    saturation rate: the highest steady-state rate divided by the mean work
+    of a request."""
     from nls_amd import serving as S
-    import torch
-    eng = S.Engine(budget_bytes=16 << 30)
-    reqs = S.poisson_requests(rate, window, seed=seed, device_inputs=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    done = eng.run(reqs)
-    wall = time.perf_counter() - t0
-    eng.close()
-    s = S.summarize(done, wall)
-    s.update({"rate_per_s": rate, "arrival_window_s": window, "wall_s": wall})
-    log("serving at %g req/s: %d requests, p50 %.3fs, p99 %.3fs" % (rate, s["requests"], s["p50_s"], s["p99_s"]))
-    return s
+    runs = []
+    for rate in rates:
+        s = S.serve(rate)
+        log("serving at %g req/s over %.1fs: %d requests, p50 %.3fs, p99 %.3fs, steady %.1f G grid-point-steps/s "
+            "(offered %.1f G)" % (rate, s["arrival_window_s"], s["requests"], s["p50_s"], s["p99_s"],
+                                 s.get("steady_grid_point_steps_per_s", 0) / 1e9,
+                                 s.get("offered_grid_point_steps_per_s", 0) / 1e9))
+        runs.append(s)
+    mean_work = S.mean_request_work()
+    cap = max(r.get("steady_grid_point_steps_per_s", 0.0) for r in runs)
+    out = {"runs": runs, "mean_request_grid_point_steps": mean_work, "capacity_grid_point_steps_per_s": cap,
+           "saturation_rate_req_s": cap / mean_work}
+    if capacity_ref:
+        out["capacity_vs_batch_engine"] = cap / capacity_ref
+    log("serving capacity %.1f G grid-point-steps/s = %.0f requests/s of the mix%s" % (
+        cap / 1e9, cap / mean_work, "" if not capacity_ref else " (%.0f%% of the batch engine)" % (100 * cap / capacity_ref)))
+    return out
 
 
 def main():
@@ -404,7 +413,6 @@ def main():
                     help="skip the FP32-vs-FP64 accuracy of the line256/square256/square2048 configurations")
     ap.add_argument("--serve-rates", default="20,50,100,200,500",
                     help="Poisson arrival rates (requests/s, comma-separated), one request-loop run each")
-    ap.add_argument("--serve-window", type=float, default=2.0, help="arrival window (s)")
     ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "hybrid", "stage", "resident", "auto"),
                     help="batch engine of the headline workload (tuned: measured on this device)")
     ap.add_argument("--grid2048-only", action="store_true",
@@ -566,7 +574,7 @@ def main():
                                                   "members": a["members"], "per_size": a["per_size"],
                                                   "note": "largest over all members; see accuracy"}
     if not args.no_serve:
-        out["serving"] = [serving(float(r), args.serve_window) for r in args.serve_rates.split(",") if r]
+        out["serving"] = serving([float(r) for r in args.serve_rates.split(",") if r], capacity_ref=value)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(members[:16], 200)
         log("cpu baseline: %.3g grid-point-steps/s (%s)" % (out["cpu_baseline"]["value"], out["cpu_baseline"]["kind"]))

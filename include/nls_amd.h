@@ -324,8 +324,18 @@ int nls_batch_tiles(const nls_batch *b);
 size_t nls_batch_workspace_bytes(const nls_batch *b);
 /* Step budget per member for the following advance calls (host array of
  * count entries): member i advances min(iters, steps[i]) steps and then
- * holds its state.  Default: unlimited.  Synchronises the stream. */
+ * holds its state.  Default: unlimited.  The member table is uploaded
+ * asynchronously, ordered on `stream` (use the stream the batch advances
+ * on); the host array may be reused on return. */
 int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream);
+/* Replaces the members of a batch of one engine (not HYBRID) and, when steps
+ * is not NULL, their budgets (else unlimited): the same checks as
+ * nls_batch_create; count may be 0 (advances then do nothing).  The tables
+ * are uploaded asynchronously, ordered on `stream`; the host waits only when
+ * the tables must grow or a captured step block must be dropped.  For a
+ * request loop that admits and retires members between advances without
+ * re-creating its batch.  No counterpart in the reference. */
+int nls_batch_update(nls_batch *b, const nls_member *members, int count, const int32_t *steps, void *stream);
 /* Advance every member by up to iters steps on device arrays in place.
  * Fused and per-stage engines: a call of at least 128 steps in which every
  * member's budget covers all the steps replays a captured block of 64 steps
@@ -355,6 +365,12 @@ int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, doubl
  * counterpart in the reference. */
 int nls_batch_energy(nls_batch *b, const void *pumping, const void *psi, const void *nres, double *out,
                      void *stream);
+/* The same two reductions written to device memory: out is a device array of
+ * 4 count doubles, enqueued on `stream` and not waited for.  No counterpart
+ * in the reference. */
+int nls_batch_observables_async(nls_batch *b, const void *psi, const void *nres, double *out, void *stream);
+int nls_batch_energy_async(nls_batch *b, const void *pumping, const void *psi, const void *nres, double *out,
+                           void *stream);
 
 #ifdef __cplusplus
 }

@@ -39,6 +39,8 @@ template <typename T> struct MemberDev {
     int n, steps;
     long long off;          // first point of the member in the packed arrays
     long long tab;          // first entry of its radial table (1-D ref), else -1
+    int id, pad;            // the member's index in the caller's batch (observables go to out[4 id ..])
+    double dV;              // volume element of its observables
     T dt;
     T w0, w1, w2, w3;       // Cartesian second-derivative weights
     Coeffs<T> c;
@@ -138,6 +140,32 @@ __global__ void __launch_bounds__(256) k_batch_stage(const BatchArgs<T> b) {
 // reservoir's stage input and the RK4 accumulators stay in registers; the
 // psi stage input sits in LDS so the stencil can read the neighbours.
 // ---------------------------------------------------------------------------
+
+// Buffer loads at a 32-bit byte offset from a buffer resource (base address,
+// size in bytes); offsets at or past the size read zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void *base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <typename T> __device__ __forceinline__ cplx_t<T> buffer_load_c(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float2 buffer_load_c<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ double2 buffer_load_c<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+template <typename T> __device__ __forceinline__ T buffer_load_r(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <> __device__ __forceinline__ float buffer_load_r<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <> __device__ __forceinline__ double buffer_load_r<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+
+// The fused kernel addresses a member's arrays with 32-bit byte offsets: it
+// holds members (and row slabs) whose complex array is smaller than 4 GiB.
+bool fused_holds(long long points, int dtype) {
+    return points * (dtype == NLS_F64 ? 16LL : 8LL) <= 0xffffffffLL;
+}
 
 template <typename T> struct FusedArgs {
     const MemberDev<T> *mem;
@@ -265,6 +293,11 @@ k_fused_step_2d(const FusedArgs<T> f) {
     C *uo = f.uo + off;
     T *uno = DYN ? f.uno + off : nullptr;
     const int tid = threadIdx.x;
+    // the region loads take 32-bit byte offsets from buffer resources over
+    // the member's arrays (fused_holds: under 4 GiB)
+    const unsigned cbytes = (unsigned)((unsigned long long)(BC == BC_SLAB ? f.slab_end + H : n) * n * sizeof(C));
+    const __amdgpu_buffer_rsrc_t ru = buffer_of(u, cbytes), rp = buffer_of(P, cbytes / 2),
+                                 rn = buffer_of(DYN ? un : P, cbytes / 2);
 
     if (f.it >= m.steps) {            // budget used up: pass the state through
 #pragma unroll
@@ -293,6 +326,8 @@ k_fused_step_2d(const FusedArgs<T> f) {
     }
     const T w0 = (T)2 * m.w0;          // centre weight of the 2-D Laplacian
     const T w[4] = {m.w0, m.w1, m.w2, m.w3};
+    T wc = w0;                         // meets c6 in one fused multiply-add: held in a vector register
+    if constexpr (sizeof(T) == 4) asm volatile("" : "+v"(wc));
     const unsigned *map = ring_map<W, HH, R, WS>.v;
 
     C uu[J], acc[J], ynext[DB ? 1 : J];
@@ -342,12 +377,17 @@ k_fused_step_2d(const FusedArgs<T> f) {
             int gx = td.x0 - H + lx, gy = td.y0 - H + ly;
             bool ok = true;
             if (BC == BC_PERIODIC) {
-                if (near) { gx = wrap_near(gx, n); gy = wrap_near(gy, n); }
-                else { gx = wrap_index(gx, n); gy = wrap_index(gy, n); }
+                if (near) {
+                    if constexpr (sizeof(T) == 4) { gx = wrap_near3(gx, n); gy = wrap_near3(gy, n); }
+                    else { gx = wrap_near(gx, n); gy = wrap_near(gy, n); }
+                } else {
+                    gx = wrap_index(gx, n);
+                    gy = wrap_index(gy, n);
+                }
             } else if (BC == BC_SLAB) {
                 // rows from 0 (the tile rows start at 4R); rows past the
                 // bottom ghost rows reach no written point within 4R rows
-                gx = near ? wrap_near(gx, n) : wrap_index(gx, n);
+                gx = near ? (sizeof(T) == 4 ? wrap_near3(gx, n) : wrap_near(gx, n)) : wrap_index(gx, n);
                 ok = gy < ylim + H;
             } else {
                 ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
@@ -355,9 +395,17 @@ k_fused_step_2d(const FusedArgs<T> f) {
             if (ok) {
                 const unsigned gi = (unsigned)(gy * n + gx);
                 inside |= 1ull << j;
-                uu[j] = u[gi];
-                cp[j] = P[gi];             // the pump term is formed at the store
-                if (DYN) nn[j] = un[gi];
+                if constexpr (sizeof(T) == 4) {
+                    uu[j] = buffer_load_c<T>(ru, gi * (unsigned)sizeof(C));
+                    cp[j] = buffer_load_r<T>(rp, gi * (unsigned)sizeof(T));   // the pump term is formed at the store
+                    if (DYN) nn[j] = buffer_load_r<T>(rn, gi * (unsigned)sizeof(T));
+                } else {
+                    // double precision keeps 64-bit addresses: its 512-thread
+                    // form has no registers to spare for the buffer resources
+                    uu[j] = u[gi];
+                    cp[j] = P[gi];
+                    if (DYN) nn[j] = un[gi];
+                }
             }
             if (!GROUPED) ya[li[j]] = uu[j];
         }
@@ -391,22 +439,30 @@ k_fused_step_2d(const FusedArgs<T> f) {
         auto rd = [&](int i) { const V2 v = ys[i]; return mk<T>(v.x, v.y); };
         const int idx = R * WS;
         const C y = rd(idx);
-        // the stencil sum as written (centre weight times y plus the
-        // neighbour sums): this kernel is bound by its arithmetic, and the
+        // the stencil sum as written (neighbour sums weighted; the
         // difference form of lap_quad costs 8 more operations per point and
-        // stage (3.4% of the step); in single precision it lowers the error
-        // of long runs by about a quarter (2048^2, 2e4 steps: 1.7e-4 on the
-        // per-stage kernels, 2.35e-4 here)
-        T lr = w0 * y.x, lm = w0 * y.y;
+        // stage, and this kernel is bound by its arithmetic).  In single
+        // precision the centre term joins the point's own factor in the
+        // right-hand side (rhs_point_centre: two products fewer); double
+        // precision adds it here.
+        T lr, lm;
+        if constexpr (sizeof(T) == 8) { lr = w0 * y.x; lm = w0 * y.y; }
 #pragma unroll
         for (int d = 1; d <= R; ++d) {
             const C a0 = rd(idx - d), a1 = rd(idx + d);
             const C b0 = rd(idx - d * WS), b1 = rd(idx + d * WS);
-            lr += w[d] * ((a0.x + a1.x) + (b0.x + b1.x));
-            lm += w[d] * ((a0.y + a1.y) + (b0.y + b1.y));
+            const T sr = (a0.x + a1.x) + (b0.x + b1.x), sm = (a0.y + a1.y) + (b0.y + b1.y);
+            if (sizeof(T) == 4 && d == 1) {
+                lr = w[1] * sr;
+                lm = w[1] * sm;
+            } else {
+                lr += w[d] * sr;
+                lm += w[d] * sm;
+            }
         }
         kn = 0;
-        rhs_point_cp<T, DYN>(c, cp[j], y, yn[j], mk<T>(lr, lm), k, kn);
+        if constexpr (sizeof(T) == 4) rhs_point_centre<T, DYN>(c, cp[j], wc, y, yn[j], mk<T>(lr, lm), k, kn);
+        else rhs_point_cp<T, DYN>(c, cp[j], y, yn[j], mk<T>(lr, lm), k, kn);
     };
 
     // stages 1..3 on the shrinking rectangles.  Stage s computes the points
@@ -881,6 +937,9 @@ static int fused_slab_run(const nls_problem *p, int rows, const T *P, const cplx
 
 int fused_slab_step(const nls_problem *p, int rows, const void *P, const void *psi_in, const void *nres_in,
                     void *psi_out, void *nres_out, hipStream_t s) {
+    if (!fused_holds((long long)(rows + 4 * (p->order - 1)) * p->n, p->dtype))
+        return fail(NLS_EUNSUPPORTED, "a slab of %d rows x %d points is too large for the fused kernel (complex "
+                    "array of 4 GiB or more)", rows, p->n);
     if (p->dtype == NLS_F64)
         return fused_slab_run<double>(p, rows, (const double *)P, (const cplx_t<double> *)psi_in,
                                       (const double *)nres_in, (cplx_t<double> *)psi_out, (double *)nres_out, s);
@@ -943,6 +1002,9 @@ int resident_advance(const nls_problem *p, int iters, const void *pumping, void 
 
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
                        hipStream_t s) {
+    if (!fused_holds((long long)p->n * p->n, p->dtype))
+        return fail(NLS_EUNSUPPORTED, "an %d x %d grid is too large for the fused kernel (complex array of 4 GiB or "
+                    "more); the per-stage engine holds it", p->n, p->n);
     if (p->dtype == NLS_F64)
         return fused_grid_run<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)ws, s);
     return fused_grid_run<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);
@@ -958,12 +1020,26 @@ struct nls_batch {
     int count;
     size_t points;                 // length of the packed arrays
     int ntiles, chunk;
-    void *d_mem = nullptr, *d_tiles = nullptr, *d_wtab = nullptr;
-    void *d_obs = nullptr;              // observables of every member (4 doubles each), allocated at first use
-    void *h_steps_pinned = nullptr;
+    // device tables (member table, tile list, member spacings, radial
+    // tables) and their capacities in bytes; they grow when an update needs
+    // more and are otherwise reused
+    void *d_mem = nullptr, *d_tiles = nullptr, *d_dx = nullptr, *d_wtab = nullptr;
+    size_t cap_mem = 0, cap_tiles = 0, cap_dx = 0, cap_wtab = 0;
+    void *d_obs = nullptr;              // observables of every member (4 doubles each)
+    size_t cap_obs = 0;
+    // pinned host staging for the asynchronous table uploads: two slots used
+    // in turn, each free again once its copy has run (event)
+    struct Stage {
+        void *host = nullptr;
+        size_t cap = 0;
+        hipEvent_t done = nullptr;
+    } stage[2];
+    int stage_next = 0;
     std::vector<unsigned char> h_mem;   // host image of the member table
-    size_t mem_stride;
+    std::vector<TileDesc> h_tiles;      // host image of the tile list
     std::vector<double> h_dx;           // grid spacing of each member
+    size_t mem_stride = 0;
+    long long tab_total = 0;            // radial table entries (1-D reference grids)
     int min_steps = 0x7fffffff;         // smallest member step budget
     int fform = FORM_TALL;              // fused engine: tile form (fused_form)
     int flags = 0;                      // NLS_FLAG_EAGER: no step-block graphs
@@ -974,8 +1050,9 @@ struct nls_batch {
     // NLS_ENGINE_HYBRID: the members the resident engine holds (part 0) and
     // the others (part 1: fused in 2-D, per-stage in 1-D), each a batch of
     // its own over the same packed arrays (member offsets unchanged); ids[k]
-    // lists part k's members by index in this batch.  Part 0 runs on `side`,
-    // forked from the caller's stream and joined back to it.
+    // lists part k's members by index in this batch, which is also the id in
+    // the part's member table.  Part 0 runs on `side`, forked from the
+    // caller's stream and joined back to it.
     nls_batch *part[2] = {nullptr, nullptr};
     std::vector<int> ids[2];
     hipStream_t side = nullptr;
@@ -993,11 +1070,14 @@ static bool is_hybrid(const nls_batch *b) { return b->part[1] != nullptr; }
 // directly.
 constexpr int BATCH_GRAPH_BLOCK = 64;
 
+// Host image of the member table.  ids: each member's index in the caller's
+// batch (NULL: its position here).
 template <typename T>
-static void fill_members(nls_batch *b, const nls_member *ms, long long *tab_total) {
+static void fill_members(nls_batch *b, const nls_member *ms, const int *ids) {
     b->mem_stride = sizeof(MemberDev<T>);
     b->h_mem.assign(sizeof(MemberDev<T>) * b->count, 0);
     MemberDev<T> *md = (MemberDev<T> *)b->h_mem.data();
+    const bool radial = b->dim == 1 && b->bc == NLS_BC_REF;
     long long tab = 0;
     for (int i = 0; i < b->count; ++i) {
         const nls_member &m = ms[i];
@@ -1006,6 +1086,10 @@ static void fill_members(nls_batch *b, const nls_member *ms, long long *tab_tota
         d.n = m.n;
         d.steps = 0x7fffffff;
         d.off = m.offset;
+        d.id = ids ? ids[i] : i;
+        // volume element of the observables: 2 pi r dr on the radial grid
+        // (the sums carry the weight i, r = i dx), dx on a line, dx^2 on a square
+        d.dV = radial ? 2.0 * M_PI * m.dx * m.dx : (b->dim == 1 ? m.dx : m.dx * m.dx);
         d.dt = (T)m.dt;
         d.c = make_coeffs<T>(m.coeffs);
         const int R = (b->order - 1) / 2;
@@ -1013,10 +1097,10 @@ static void fill_members(nls_batch *b, const nls_member *ms, long long *tab_tota
         for (int q = 0; q <= R; ++q) w[q] = (T)stencil_s2(b->order, q, m.dx);
         d.w0 = w[0]; d.w1 = w[1]; d.w2 = w[2]; d.w3 = w[3];
         d.tab = -1;
-        if (b->dim == 1 && b->bc == NLS_BC_REF) { d.tab = tab; tab += (long long)m.n * b->order; }
+        if (radial) { d.tab = tab; tab += (long long)m.n * b->order; }
         md[i] = d;
     }
-    *tab_total = tab;
+    b->tab_total = tab;
 }
 
 template <typename T>
@@ -1125,6 +1209,8 @@ static int batch_steps(nls_batch *b, int it0, int iters, const T *P, cplx_t<T> *
 // precision.  Each thread accumulates a fixed strided slice; the wavefront
 // reduces with lane shuffles (xor 32 .. 1) and the four wavefronts' sums are
 // added in order, so a member's values do not depend on the rest of the batch.
+// Member i's values go to out[4 id ..], the norm and population times its
+// volume element dV.
 template <typename T>
 __global__ void __launch_bounds__(256) k_batch_obs(const MemberDev<T> *mem, int dim, int radial,
                                                    const cplx_t<T> *psi, const T *nres, double *out) {
@@ -1160,7 +1246,11 @@ __global__ void __launch_bounds__(256) k_batch_obs(const MemberDev<T> *mem, int 
         for (int w = 1; w < 4; ++w) {
             o[0] += part[w][0]; o[1] += part[w][1]; o[2] = fmax(o[2], part[w][2]); o[3] += part[w][3];
         }
-        for (int q = 0; q < 4; ++q) out[(size_t)blockIdx.x * 4 + q] = o[q];
+        double *dst = out + (size_t)m.id * 4;
+        dst[0] = __dmul_rn(o[0], m.dV);
+        dst[1] = __dmul_rn(o[1], m.dV);
+        dst[2] = o[2];
+        dst[3] = o[3];
     }
 }
 
@@ -1170,7 +1260,8 @@ __global__ void __launch_bounds__(256) k_batch_obs(const MemberDev<T> *mem, int 
 // table on the 1-D reference grid), FP64 accumulators, wavefront shuffles and
 // the four wavefronts added in order.  out[4 i + 1..3] = kinetic
 // -Re(conj psi L psi) w, (c5/2) |psi|^4 w, c6 r |psi|^2 w summed (w = i on the
-// radial grid, else 1; the host applies the volume element).
+// radial grid, else 1), each times the volume element dV, and out[4 id] their
+// sum (products and sums rounded one by one, no fused multiply-add).
 template <typename T, int ORDER, int BC, int DIM>
 __global__ void __launch_bounds__(256) k_batch_energy(const MemberDev<T> *mem, const T *wtab,
                                                       const cplx_t<T> *psi, const T *P, const T *nres,
@@ -1207,10 +1298,14 @@ __global__ void __launch_bounds__(256) k_batch_energy(const MemberDev<T> *mem, c
         double o[3] = {part[0][0], part[0][1], part[0][2]};
         for (int k = 1; k < 4; ++k)
             for (int q = 0; q < 3; ++q) o[q] += part[k][q];
-        out[(size_t)blockIdx.x * 4 + 0] = 0.0;
-        out[(size_t)blockIdx.x * 4 + 1] = o[0];
-        out[(size_t)blockIdx.x * 4 + 2] = 0.5 * (double)m.c.c5 * o[1];
-        out[(size_t)blockIdx.x * 4 + 3] = (double)m.c.c6 * o[2];
+        double *dst = out + (size_t)m.id * 4;
+        const double e1 = __dmul_rn(o[0], m.dV);
+        const double e2 = __dmul_rn(__dmul_rn(__dmul_rn(0.5, (double)m.c.c5), o[1]), m.dV);
+        const double e3 = __dmul_rn(__dmul_rn((double)m.c.c6, o[2]), m.dV);
+        dst[0] = __dadd_rn(__dadd_rn(e1, e2), e3);
+        dst[1] = e1;
+        dst[2] = e2;
+        dst[3] = e3;
     }
 }
 
@@ -1297,6 +1392,213 @@ static int batch_advance(nls_batch *b, int iters, const T *P, cplx_t<T> *psi, T 
     return rest ? batch_steps<T>(b, iters - rest, rest, P, psi, nres, ws, s) : NLS_OK;
 }
 
+// ---- batch tables: planning and asynchronous upload -----------------------
+
+// Member checks shared by create and update: sizes, spacings, offsets, one
+// reservoir mode, no overlap in the packed arrays; *points = the packed
+// length the members need.
+static int check_members(int dim, int order, bool dyn, const nls_member *members, int count, size_t *points) {
+    size_t pts = 0;
+    std::vector<std::pair<long long, long long>> iv;
+    for (int i = 0; i < count; ++i) {
+        const nls_member &m = members[i];
+        if (m.n < order) return fail(NLS_EINVAL, "member %d: n=%d smaller than the stencil width", i, m.n);
+        if (dim == 2 && m.n > 46340) return fail(NLS_EINVAL, "member %d: n too large", i);
+        if (!(m.dx > 0.0)) return fail(NLS_EINVAL, "member %d: dx must be positive", i);
+        if (m.offset < 0) return fail(NLS_EINVAL, "member %d: negative offset", i);
+        if ((m.coeffs[10] != 0.0) != dyn) return fail(NLS_EINVAL, "members mix reservoir modes");
+        const long long np = dim == 1 ? (long long)m.n : (long long)m.n * m.n;
+        pts = std::max(pts, (size_t)(m.offset + np));
+        iv.push_back({m.offset, m.offset + np});
+    }
+    std::sort(iv.begin(), iv.end());
+    for (size_t i = 1; i < iv.size(); ++i)
+        if (iv[i].first < iv[i - 1].second) return fail(NLS_EINVAL, "members overlap in the packed arrays");
+    *points = pts;
+    return NLS_OK;
+}
+
+// Host images of the member table, the tile list and the spacings of
+// `members` (ids: their indices in the caller's batch, NULL for 0..count-1).
+static void plan(nls_batch *b, const nls_member *members, int count, const int *ids, size_t points) {
+    b->count = count;
+    b->points = points;
+    b->h_dx.assign(count, 0.0);
+    for (int i = 0; i < count; ++i) b->h_dx[i] = members[i].dx;
+    if (b->dtype == NLS_F64) fill_members<double>(b, members, ids);
+    else fill_members<float>(b, members, ids);
+    b->min_steps = 0x7fffffff;
+    // tiles, member by member, row-major inside a member
+    int tx = 64, ty = BATCH_STAGE_TY;
+    if (b->engine == NLS_ENGINE_FUSED) {
+        long long tiles16 = 0, tiles32 = 0;
+        for (int i = 0; i < count; ++i) {
+            tiles16 += tiles_of(members[i].n, members[i].n, 16);
+            tiles32 += tiles_of(members[i].n, members[i].n, 32);
+        }
+        b->fform = fused_form(b->dtype, b->order, b->bc, tiles16, tiles32);
+        ty = fused_ty(b->order, b->fform);
+    }
+    if (b->dim == 1) { tx = TILE1; ty = 1; }
+    b->h_tiles.clear();
+    if (b->engine != NLS_ENGINE_RESIDENT)
+        for (int i = 0; i < count; ++i) {
+            const int n = members[i].n;
+            const int nyt = b->dim == 1 ? 1 : (n + ty - 1) / ty, nxt = (n + tx - 1) / tx;
+            for (int y = 0; y < nyt; ++y)
+                for (int x = 0; x < nxt; ++x) b->h_tiles.push_back({i, x * tx, b->dim == 1 ? 0 : y * ty, 0});
+        }
+    b->ntiles = (int)b->h_tiles.size();
+    b->chunk = (b->ntiles + 7) / 8;
+}
+
+// Device buffer of at least `need` bytes; a larger one replaces it after the
+// work queued on s (the only stream that reads it) has run.
+static int grow(void **p, size_t *cap, size_t need, hipStream_t s, const char *what) {
+    if (need <= *cap) return NLS_OK;
+    int rc = NLS_OK;
+    if (*p) {
+        rc = hip_check(hipStreamSynchronize(s), what);
+        (void)hipFree(*p);
+        *p = nullptr;
+    }
+    const size_t c = std::max(need, 2 * *cap);
+    *cap = 0;
+    if (!rc) rc = hip_check(hipMalloc(p, c), what);
+    if (!rc) *cap = c;
+    return rc;
+}
+
+// Copies the member table -- and with `tables` the tile list and spacings,
+// then rebuilds the radial tables -- to the device, enqueued on s through the
+// next pinned staging slot (the host waits only if that slot's previous copy
+// has not run yet).
+static int upload(nls_batch *b, hipStream_t s, bool tables) {
+    const size_t nm = b->h_mem.size();
+    if (nm == 0) return NLS_OK;
+    const size_t nt = tables ? b->h_tiles.size() * sizeof(TileDesc) : 0;
+    const size_t nd = tables ? b->h_dx.size() * sizeof(double) : 0;
+    const size_t rs = b->dtype == NLS_F64 ? 8 : 4;
+    int rc = grow(&b->d_mem, &b->cap_mem, nm, s, "batch members");
+    if (!rc && nt) rc = grow(&b->d_tiles, &b->cap_tiles, nt, s, "batch tiles");
+    if (!rc && nd) rc = grow(&b->d_dx, &b->cap_dx, nd, s, "batch spacings");
+    if (!rc && tables && b->tab_total > 0) rc = grow(&b->d_wtab, &b->cap_wtab, b->tab_total * rs, s, "radial tables");
+    if (rc) return rc;
+    nls_batch::Stage &st = b->stage[b->stage_next];
+    b->stage_next ^= 1;
+    if (!st.done) rc = hip_check(hipEventCreateWithFlags(&st.done, hipEventDisableTiming), "staging event");
+    else rc = hip_check(hipEventSynchronize(st.done), "staging slot");
+    if (rc) return rc;
+    const size_t om = 0, ot = align256b(nm), od = ot + align256b(nt), need = od + nd;
+    if (need > st.cap) {
+        if (st.host) (void)hipHostFree(st.host);
+        st.host = nullptr;
+        st.cap = 0;
+        const size_t c = std::max(need, 2 * st.cap);
+        if ((rc = hip_check(hipHostMalloc(&st.host, c, hipHostMallocDefault), "staging buffer"))) return rc;
+        st.cap = c;
+    }
+    char *h = (char *)st.host;
+    memcpy(h + om, b->h_mem.data(), nm);
+    if (nt) memcpy(h + ot, b->h_tiles.data(), nt);
+    if (nd) memcpy(h + od, b->h_dx.data(), nd);
+    rc = hip_check(hipMemcpyAsync(b->d_mem, h + om, nm, hipMemcpyHostToDevice, s), "upload members");
+    if (!rc && nt) rc = hip_check(hipMemcpyAsync(b->d_tiles, h + ot, nt, hipMemcpyHostToDevice, s), "upload tiles");
+    if (!rc && nd) rc = hip_check(hipMemcpyAsync(b->d_dx, h + od, nd, hipMemcpyHostToDevice, s), "upload spacings");
+    if (!rc) rc = hip_check(hipEventRecord(st.done, s), "staging event");
+    if (!rc && tables && b->tab_total > 0) {
+        if (b->dtype == NLS_F64)
+            hipLaunchKernelGGL(k_batch_radial_tables<double>, dim3(b->count), dim3(256), 0, s,
+                               (const MemberDev<double> *)b->d_mem, b->count, b->order, (const double *)b->d_dx,
+                               (double *)b->d_wtab);
+        else
+            hipLaunchKernelGGL(k_batch_radial_tables<float>, dim3(b->count), dim3(256), 0, s,
+                               (const MemberDev<float> *)b->d_mem, b->count, b->order, (const double *)b->d_dx,
+                               (float *)b->d_wtab);
+        rc = hip_check(hipGetLastError(), "radial tables");
+    }
+    return rc;
+}
+
+// Budgets into the host image of the member table.
+static int set_budgets(nls_batch *b, const int32_t *steps) {
+    for (int i = 0; i < b->count; ++i)
+        if (steps[i] < 0) return fail(NLS_EINVAL, "member %d: negative step budget", i);
+    b->min_steps = 0x7fffffff;
+    for (int i = 0; i < b->count; ++i) {
+        if (b->dtype == NLS_F64) set_member_steps<double>(b, i, steps[i]);
+        else set_member_steps<float>(b, i, steps[i]);
+        b->min_steps = std::min(b->min_steps, (int)steps[i]);
+    }
+    return NLS_OK;
+}
+
+// A batch of one engine (not hybrid) over checked members; ids as plan().
+static int create_single(int dim, int order, int bc, int dtype, int engine, bool dyn, const nls_member *members,
+                         int count, const int *ids, size_t points, nls_batch **out) {
+    nls_batch *b = new nls_batch();
+    b->dim = dim; b->order = order; b->bc = bc; b->dtype = dtype; b->engine = engine; b->dyn = dyn;
+    plan(b, members, count, ids, points);
+    int rc = upload(b, nullptr, true);
+    if (!rc) rc = hip_check(hipStreamSynchronize(nullptr), "batch tables");
+    if (rc) { nls_batch_destroy(b); return rc; }
+    *out = b;
+    return NLS_OK;
+}
+
+// Observables / energy of every member into device memory out (4 doubles a
+// member at 4 id), enqueued on s.
+template <typename T>
+static void launch_obs_t(nls_batch *b, const void *psi, const void *nres, double *out, hipStream_t s) {
+    const int radial = (b->dim == 1 && b->bc == NLS_BC_REF) ? 1 : 0;
+    hipLaunchKernelGGL((k_batch_obs<T>), dim3(b->count), dim3(256), 0, s, (const MemberDev<T> *)b->d_mem, b->dim,
+                       radial, (const cplx_t<T> *)psi, b->dyn ? (const T *)nres : nullptr, out);
+}
+
+static int launch_obs(nls_batch *b, const void *psi, const void *nres, double *out, hipStream_t s) {
+    if (is_hybrid(b)) {
+        for (nls_batch *q : b->part) {
+            const int rc = launch_obs(q, psi, nres, out, s);
+            if (rc) return rc;
+        }
+        return NLS_OK;
+    }
+    if (b->count == 0) return NLS_OK;
+    if (b->dtype == NLS_F64) launch_obs_t<double>(b, psi, nres, out, s);
+    else launch_obs_t<float>(b, psi, nres, out, s);
+    return hip_check(hipGetLastError(), "observables launch");
+}
+
+static int launch_energy(nls_batch *b, const void *P, const void *psi, const void *nres, double *out, hipStream_t s) {
+    if (is_hybrid(b)) {
+        for (nls_batch *q : b->part) {
+            const int rc = launch_energy(q, P, psi, nres, out, s);
+            if (rc) return rc;
+        }
+        return NLS_OK;
+    }
+    if (b->count == 0) return NLS_OK;
+    if (b->dtype == NLS_F64)
+        launch_batch_energy<double>(b, (const double *)P, (const double2 *)psi, b->dyn ? (const double *)nres : nullptr,
+                                    out, s);
+    else
+        launch_batch_energy<float>(b, (const float *)P, (const float2 *)psi, b->dyn ? (const float *)nres : nullptr,
+                                   out, s);
+    return hip_check(hipGetLastError(), "energy launch");
+}
+
+// Runs launch (into the batch's device buffer) and downloads the 4 count
+// doubles to the host array out; blocks until done.
+template <typename F> static int obs_blocking(nls_batch *b, double *out, hipStream_t s, F launch) {
+    const size_t bytes = (size_t)b->count * 4 * sizeof(double);
+    if (bytes == 0) return NLS_OK;
+    int rc = grow(&b->d_obs, &b->cap_obs, bytes, s, "observables buffer");
+    if (!rc) rc = launch((double *)b->d_obs);
+    if (!rc) rc = hip_check(hipMemcpyAsync(out, b->d_obs, bytes, hipMemcpyDeviceToHost, s), "observables download");
+    if (!rc) rc = hip_check(hipStreamSynchronize(s), "observables sync");
+    return rc;
+}
+
 extern "C" {
 
 int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nls_member *members, int count,
@@ -1311,7 +1613,9 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
     const bool res_built = !(dim == 2 && dtype == NLS_F64 && order == 7);
     int fit = 0;
     for (int i = 0; i < count; ++i) fit += res_built && members[i].n <= res_max;
-    const int rest_engine = dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
+    bool holds = true;                 // every member fits the fused kernel's 32-bit offsets
+    for (int i = 0; i < count; ++i) holds = holds && fused_holds((long long)members[i].n * members[i].n, dtype);
+    const int rest_engine = dim == 2 && holds ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
     if (engine == NLS_ENGINE_AUTO) engine = fit == count ? NLS_ENGINE_RESIDENT : fit > 0 ? NLS_ENGINE_HYBRID : rest_engine;
     if (engine == NLS_ENGINE_HYBRID && (fit == 0 || fit == count))      // one part only
         engine = fit == count ? NLS_ENGINE_RESIDENT : rest_engine;
@@ -1326,103 +1630,37 @@ int nls_batch_create(int dim, int order, int bc, int dtype, int engine, const nl
                 return fail(NLS_EUNSUPPORTED, "member %d: n=%d does not fit one workgroup (resident engine: n <= %d)",
                             i, members[i].n, res_max);
     if (engine == NLS_ENGINE_FUSED && dim != 2) return fail(NLS_EUNSUPPORTED, "the fused engine is 2-D only");
+    if (engine == NLS_ENGINE_FUSED && !holds)
+        return fail(NLS_EUNSUPPORTED, "the fused engine holds members whose complex array is under 4 GiB");
     const bool dyn = members[0].coeffs[10] != 0.0;
     size_t points = 0;
-    for (int i = 0; i < count; ++i) {
-        const nls_member &m = members[i];
-        if (m.n < order) return fail(NLS_EINVAL, "member %d: n=%d smaller than the stencil width", i, m.n);
-        if (dim == 2 && m.n > 46340) return fail(NLS_EINVAL, "member %d: n too large", i);
-        if (!(m.dx > 0.0)) return fail(NLS_EINVAL, "member %d: dx must be positive", i);
-        if (m.offset < 0) return fail(NLS_EINVAL, "member %d: negative offset", i);
-        if ((m.coeffs[10] != 0.0) != dyn) return fail(NLS_EINVAL, "members mix reservoir modes");
-        const size_t np = dim == 1 ? (size_t)m.n : (size_t)m.n * m.n;
-        points = std::max(points, (size_t)m.offset + np);
-    }
-    // members must not overlap in the packed arrays
-    {
-        std::vector<std::pair<long long, long long>> iv;
-        for (int i = 0; i < count; ++i) {
-            const long long np = dim == 1 ? members[i].n : (long long)members[i].n * members[i].n;
-            iv.push_back({members[i].offset, members[i].offset + np});
-        }
-        std::sort(iv.begin(), iv.end());
-        for (size_t i = 1; i < iv.size(); ++i)
-            if (iv[i].first < iv[i - 1].second) return fail(NLS_EINVAL, "members overlap in the packed arrays");
-    }
-    if (engine == NLS_ENGINE_HYBRID) {
-        nls_batch *b = new nls_batch();
-        b->dim = dim; b->order = order; b->bc = bc; b->dtype = dtype; b->engine = engine; b->dyn = dyn;
-        b->count = count; b->points = points;
-        std::vector<nls_member> ms[2];
-        for (int i = 0; i < count; ++i) {
-            b->h_dx.push_back(members[i].dx);
-            const int k = res_built && members[i].n <= res_max ? 0 : 1;
-            b->ids[k].push_back(i);
-            ms[k].push_back(members[i]);
-        }
-        int rc = nls_batch_create(dim, order, bc, dtype, NLS_ENGINE_RESIDENT, ms[0].data(), (int)ms[0].size(),
-                                  &b->part[0]);
-        if (!rc) rc = nls_batch_create(dim, order, bc, dtype, rest_engine, ms[1].data(), (int)ms[1].size(), &b->part[1]);
-        if (!rc) rc = hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hybrid stream");
-        if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "hybrid event");
-        if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming), "hybrid event");
-        if (rc) { nls_batch_destroy(b); return rc; }
-        b->ntiles = b->part[1]->ntiles;
-        *out = b;
-        return NLS_OK;
-    }
+    int rc = check_members(dim, order, dyn, members, count, &points);
+    if (rc) return rc;
+    if (engine != NLS_ENGINE_HYBRID)
+        return create_single(dim, order, bc, dtype, engine, dyn, members, count, nullptr, points, out);
     nls_batch *b = new nls_batch();
     b->dim = dim; b->order = order; b->bc = bc; b->dtype = dtype; b->engine = engine; b->dyn = dyn;
     b->count = count; b->points = points;
-    for (int i = 0; i < count; ++i) b->h_dx.push_back(members[i].dx);
-    long long tab_total = 0;
-    if (dtype == NLS_F64) fill_members<double>(b, members, &tab_total);
-    else fill_members<float>(b, members, &tab_total);
-    // tiles, member by member, row-major inside a member
-    std::vector<TileDesc> tiles;
-    int tx = 64, ty = BATCH_STAGE_TY;
-    if (engine == NLS_ENGINE_FUSED) {
-        long long tiles16 = 0, tiles32 = 0;
-        for (int i = 0; i < count; ++i) {
-            tiles16 += tiles_of(members[i].n, members[i].n, 16);
-            tiles32 += tiles_of(members[i].n, members[i].n, 32);
-        }
-        b->fform = fused_form(dtype, order, bc, tiles16, tiles32);
-        ty = fused_ty(order, b->fform);
-    }
-    if (dim == 1) { tx = TILE1; ty = 1; }
+    std::vector<nls_member> ms[2];
     for (int i = 0; i < count; ++i) {
-        const int n = members[i].n;
-        const int nyt = dim == 1 ? 1 : (n + ty - 1) / ty, nxt = (n + tx - 1) / tx;
-        for (int y = 0; y < nyt; ++y)
-            for (int x = 0; x < nxt; ++x) tiles.push_back({i, x * tx, dim == 1 ? 0 : y * ty, 0});
+        b->h_dx.push_back(members[i].dx);
+        const int k = res_built && members[i].n <= res_max ? 0 : 1;
+        b->ids[k].push_back(i);
+        ms[k].push_back(members[i]);
     }
-    b->ntiles = (int)tiles.size();
-    b->chunk = (b->ntiles + 7) / 8;
-    int rc = hip_check(hipMalloc(&b->d_mem, b->h_mem.size()), "batch members");
-    if (!rc) rc = hip_check(hipMalloc(&b->d_tiles, tiles.size() * sizeof(TileDesc)), "batch tiles");
-    if (!rc) rc = hip_check(hipMemcpy(b->d_mem, b->h_mem.data(), b->h_mem.size(), hipMemcpyHostToDevice), "upload");
-    if (!rc) rc = hip_check(hipMemcpy(b->d_tiles, tiles.data(), tiles.size() * sizeof(TileDesc), hipMemcpyHostToDevice), "upload");
-    if (!rc && tab_total > 0) {
-        const size_t rs = dtype == NLS_F64 ? 8 : 4;
-        void *ddx = nullptr;
-        std::vector<double> dx(count);
-        for (int i = 0; i < count; ++i) dx[i] = members[i].dx;
-        rc = hip_check(hipMalloc(&b->d_wtab, tab_total * rs), "radial tables");
-        if (!rc) rc = hip_check(hipMalloc(&ddx, count * sizeof(double)), "dx");
-        if (!rc) rc = hip_check(hipMemcpy(ddx, dx.data(), count * sizeof(double), hipMemcpyHostToDevice), "upload");
-        if (!rc) {
-            if (dtype == NLS_F64)
-                hipLaunchKernelGGL(k_batch_radial_tables<double>, dim3(count), dim3(256), 0, 0,
-                                   (const MemberDev<double> *)b->d_mem, count, order, (const double *)ddx, (double *)b->d_wtab);
-            else
-                hipLaunchKernelGGL(k_batch_radial_tables<float>, dim3(count), dim3(256), 0, 0,
-                                   (const MemberDev<float> *)b->d_mem, count, order, (const double *)ddx, (float *)b->d_wtab);
-            rc = hip_check(hipDeviceSynchronize(), "radial tables");
-        }
-        if (ddx) (void)hipFree(ddx);
-    }
+    size_t pts[2] = {0, 0};
+    for (int k = 0; k < 2 && !rc; ++k) rc = check_members(dim, order, dyn, ms[k].data(), (int)ms[k].size(), &pts[k]);
+    if (!rc)
+        rc = create_single(dim, order, bc, dtype, NLS_ENGINE_RESIDENT, dyn, ms[0].data(), (int)ms[0].size(),
+                           b->ids[0].data(), pts[0], &b->part[0]);
+    if (!rc)
+        rc = create_single(dim, order, bc, dtype, rest_engine, dyn, ms[1].data(), (int)ms[1].size(), b->ids[1].data(),
+                           pts[1], &b->part[1]);
+    if (!rc) rc = hip_check(hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking), "hybrid stream");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming), "hybrid event");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->join_ev, hipEventDisableTiming), "hybrid event");
     if (rc) { nls_batch_destroy(b); return rc; }
+    b->ntiles = b->part[1]->ntiles;
     *out = b;
     return NLS_OK;
 }
@@ -1435,10 +1673,15 @@ int nls_batch_destroy(nls_batch *b) {
     if (b->join_ev) (void)hipEventDestroy(b->join_ev);
     (void)drop_graph(b);
     if (b->capture) (void)hipStreamDestroy(b->capture);
-    if (b->d_mem) (void)hipFree(b->d_mem);
-    if (b->d_tiles) (void)hipFree(b->d_tiles);
-    if (b->d_wtab) (void)hipFree(b->d_wtab);
-    if (b->d_obs) (void)hipFree(b->d_obs);
+    for (nls_batch::Stage &st : b->stage) {
+        if (st.done) {
+            (void)hipEventSynchronize(st.done);
+            (void)hipEventDestroy(st.done);
+        }
+        if (st.host) (void)hipHostFree(st.host);
+    }
+    for (void *p : {b->d_mem, b->d_tiles, b->d_dx, b->d_wtab, b->d_obs})
+        if (p) (void)hipFree(p);
     delete b;
     return NLS_OK;
 }
@@ -1457,12 +1700,44 @@ size_t nls_batch_workspace_bytes(const nls_batch *b) {
     return 3 * cb + (b->dyn ? 3 * rb : 0);
 }
 
+int nls_batch_update(nls_batch *b, const nls_member *members, int count, const int32_t *steps, void *stream) {
+    if (!b) return fail(NLS_EINVAL, "NULL batch");
+    if (count < 0 || (count > 0 && !members)) return fail(NLS_EINVAL, "bad member list");
+    if (is_hybrid(b))
+        return fail(NLS_EUNSUPPORTED, "a hybrid batch splits its members between engines at creation; update "
+                    "batches of one engine");
+    if (b->engine == NLS_ENGINE_RESIDENT)
+        for (int i = 0; i < count; ++i)
+            if (members[i].n > (b->dim == 1 ? 4096 : 64))
+                return fail(NLS_EUNSUPPORTED, "member %d: n=%d does not fit one workgroup", i, members[i].n);
+    if (b->engine == NLS_ENGINE_FUSED)
+        for (int i = 0; i < count; ++i)
+            if (!fused_holds((long long)members[i].n * members[i].n, b->dtype))
+                return fail(NLS_EUNSUPPORTED, "member %d is too large for the fused engine", i);
+    size_t points = 0;
+    int rc = check_members(b->dim, b->order, b->dyn, members, count, &points);
+    if (rc) return rc;
+    if (steps)
+        for (int i = 0; i < count; ++i)
+            if (steps[i] < 0) return fail(NLS_EINVAL, "member %d: negative step budget", i);
+    hipStream_t s = (hipStream_t)stream;
+    if (b->graph) {
+        // a captured block holds the old launch shapes; it may be in flight on s
+        if ((rc = hip_check(hipStreamSynchronize(s), "update"))) return rc;
+        (void)hipGraphExecDestroy(b->graph);
+        b->graph = nullptr;
+    }
+    plan(b, members, count, nullptr, points);
+    if (steps && (rc = set_budgets(b, steps))) return rc;
+    return upload(b, s, true);
+}
+
 int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream) {
     if (!b || !steps) return fail(NLS_EINVAL, "NULL argument");
     for (int i = 0; i < b->count; ++i)
         if (steps[i] < 0) return fail(NLS_EINVAL, "member %d: negative step budget", i);
     if (is_hybrid(b)) {
-        // the caller's stream has joined every launch of the side stream
+        // uploads on the caller's stream, which the side stream follows
         for (int k = 0; k < 2; ++k) {
             std::vector<int32_t> st;
             for (int i : b->ids[k]) st.push_back(steps[i]);
@@ -1471,99 +1746,36 @@ int nls_batch_set_steps(nls_batch *b, const int32_t *steps, void *stream) {
         }
         return NLS_OK;
     }
-    b->min_steps = 0x7fffffff;
-    for (int i = 0; i < b->count; ++i) {
-        if (b->dtype == NLS_F64) set_member_steps<double>(b, i, steps[i]);
-        else set_member_steps<float>(b, i, steps[i]);
-        if (steps[i] < b->min_steps) b->min_steps = steps[i];
-    }
-    hipStream_t s = (hipStream_t)stream;
-    // the member table may still be read by queued launches: wait, then copy
-    int rc = hip_check(hipStreamSynchronize(s), "set_steps sync");
-    if (!rc) rc = hip_check(hipMemcpy(b->d_mem, b->h_mem.data(), b->h_mem.size(), hipMemcpyHostToDevice), "set_steps");
-    return rc;
+    const int rc = set_budgets(b, steps);
+    return rc ? rc : upload(b, (hipStream_t)stream, false);
 }
 
 int nls_batch_observables(nls_batch *b, const void *psi, const void *nres, double *out, void *stream) {
     if (!b || !psi || !out) return fail(NLS_EINVAL, "NULL argument");
-    if (is_hybrid(b)) {
-        for (int k = 0; k < 2; ++k) {
-            std::vector<double> o(b->ids[k].size() * 4);
-            const int rc = nls_batch_observables(b->part[k], psi, nres, o.data(), stream);
-            if (rc) return rc;
-            for (size_t q = 0; q < b->ids[k].size(); ++q)
-                for (int c = 0; c < 4; ++c) out[4 * b->ids[k][q] + c] = o[4 * q + c];
-        }
-        return NLS_OK;
-    }
+    if (b->dyn && !nres) return fail(NLS_EINVAL, "NULL reservoir buffer");
     hipStream_t s = (hipStream_t)stream;
-    const int radial = (b->dim == 1 && b->bc == NLS_BC_REF) ? 1 : 0;
-    int rc = NLS_OK;
-    if (!b->d_obs) rc = hip_check(hipMalloc(&b->d_obs, (size_t)b->count * 4 * sizeof(double)), "observables buffer");
-    if (rc) return rc;
-    void *d_out = b->d_obs;
-    if (b->dtype == NLS_F64)
-        hipLaunchKernelGGL((k_batch_obs<double>), dim3(b->count), dim3(256), 0, s, (const MemberDev<double> *)b->d_mem,
-                           b->dim, radial, (const double2 *)psi, b->dyn ? (const double *)nres : nullptr,
-                           (double *)d_out);
-    else
-        hipLaunchKernelGGL((k_batch_obs<float>), dim3(b->count), dim3(256), 0, s, (const MemberDev<float> *)b->d_mem,
-                           b->dim, radial, (const float2 *)psi, b->dyn ? (const float *)nres : nullptr,
-                           (double *)d_out);
-    rc = hip_check(hipGetLastError(), "observables launch");
-    if (!rc) rc = hip_check(hipMemcpyAsync(out, d_out, (size_t)b->count * 4 * sizeof(double), hipMemcpyDeviceToHost, s),
-                            "observables download");
-    if (!rc) rc = hip_check(hipStreamSynchronize(s), "observables sync");
-    if (rc) return rc;
-    for (int i = 0; i < b->count; ++i) {
-        // volume element, as nls_observables: 2 pi r dr on the radial grid
-        // (the kernel summed with weight i, r = i dx), dx on a line, dx^2 on a square
-        const double dx = b->h_dx[i];
-        const double dV = radial ? 2.0 * M_PI * dx * dx : (b->dim == 1 ? dx : dx * dx);
-        out[4 * i + 0] *= dV;
-        out[4 * i + 1] *= dV;
-    }
-    return NLS_OK;
+    return obs_blocking(b, out, s, [&](double *d) { return launch_obs(b, psi, nres, d, s); });
+}
+
+int nls_batch_observables_async(nls_batch *b, const void *psi, const void *nres, double *out, void *stream) {
+    if (!b || !psi || !out) return fail(NLS_EINVAL, "NULL argument");
+    if (b->dyn && !nres) return fail(NLS_EINVAL, "NULL reservoir buffer");
+    return launch_obs(b, psi, nres, out, (hipStream_t)stream);
 }
 
 int nls_batch_energy(nls_batch *b, const void *pumping, const void *psi, const void *nres, double *out,
                      void *stream) {
     if (!b || !pumping || !psi || !out) return fail(NLS_EINVAL, "NULL argument");
     if (b->dyn && !nres) return fail(NLS_EINVAL, "NULL reservoir buffer");
-    if (is_hybrid(b)) {
-        for (int k = 0; k < 2; ++k) {
-            std::vector<double> o(b->ids[k].size() * 4);
-            const int rc = nls_batch_energy(b->part[k], pumping, psi, nres, o.data(), stream);
-            if (rc) return rc;
-            for (size_t q = 0; q < b->ids[k].size(); ++q)
-                for (int c = 0; c < 4; ++c) out[4 * b->ids[k][q] + c] = o[4 * q + c];
-        }
-        return NLS_OK;
-    }
     hipStream_t s = (hipStream_t)stream;
-    int rc = NLS_OK;
-    if (!b->d_obs) rc = hip_check(hipMalloc(&b->d_obs, (size_t)b->count * 4 * sizeof(double)), "observables buffer");
-    if (rc) return rc;
-    double *d_out = (double *)b->d_obs;
-    if (b->dtype == NLS_F64)
-        launch_batch_energy<double>(b, (const double *)pumping, (const double2 *)psi,
-                                    b->dyn ? (const double *)nres : nullptr, d_out, s);
-    else
-        launch_batch_energy<float>(b, (const float *)pumping, (const float2 *)psi,
-                                   b->dyn ? (const float *)nres : nullptr, d_out, s);
-    rc = hip_check(hipGetLastError(), "energy launch");
-    if (!rc) rc = hip_check(hipMemcpyAsync(out, d_out, (size_t)b->count * 4 * sizeof(double), hipMemcpyDeviceToHost, s),
-                            "energy download");
-    if (!rc) rc = hip_check(hipStreamSynchronize(s), "energy sync");
-    if (rc) return rc;
-    const int radial = (b->dim == 1 && b->bc == NLS_BC_REF) ? 1 : 0;
-    for (int i = 0; i < b->count; ++i) {
-        const double dx = b->h_dx[i];
-        const double dV = radial ? 2.0 * M_PI * dx * dx : (b->dim == 1 ? dx : dx * dx);
-        for (int c = 1; c < 4; ++c) out[4 * i + c] *= dV;
-        out[4 * i + 0] = out[4 * i + 1] + out[4 * i + 2] + out[4 * i + 3];
-    }
-    return NLS_OK;
+    return obs_blocking(b, out, s, [&](double *d) { return launch_energy(b, pumping, psi, nres, d, s); });
+}
+
+int nls_batch_energy_async(nls_batch *b, const void *pumping, const void *psi, const void *nres, double *out,
+                           void *stream) {
+    if (!b || !pumping || !psi || !out) return fail(NLS_EINVAL, "NULL argument");
+    if (b->dyn && !nres) return fail(NLS_EINVAL, "NULL reservoir buffer");
+    return launch_energy(b, pumping, psi, nres, out, (hipStream_t)stream);
 }
 
 int nls_batch_prepare(nls_batch *b, int iters, const void *pumping, void *psi, void *nres, void *workspace,
@@ -1574,7 +1786,7 @@ int nls_batch_prepare(nls_batch *b, int iters, const void *pumping, void *psi, v
     if (workspace_bytes < need || (need && !workspace))
         return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
     if (is_hybrid(b)) return nls_batch_prepare(b->part[1], iters, pumping, psi, nres, workspace, workspace_bytes);
-    if (!batch_graphs_apply(b, iters)) return NLS_OK;
+    if (b->count == 0 || !batch_graphs_apply(b, iters)) return NLS_OK;
     if (b->dtype == NLS_F64)
         return batch_graph<double>(b, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)workspace);
     return batch_graph<float>(b, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)workspace);
@@ -1597,7 +1809,7 @@ int nls_batch_advance(nls_batch *b, int iters, const void *pumping, void *psi, v
     const size_t need = nls_batch_workspace_bytes(b);
     if (workspace_bytes < need || (need && !workspace))
         return fail(NLS_EINVAL, "workspace of %zu bytes is smaller than the %zu required", workspace_bytes, need);
-    if (iters == 0) return NLS_OK;
+    if (iters == 0 || b->count == 0) return NLS_OK;
     hipStream_t s = (hipStream_t)stream;
     if (is_hybrid(b)) {
         // resident part on the side stream, the rest on the caller's, joined

@@ -14,21 +14,6 @@
 
 namespace nls {
 
-static thread_local char g_err[512] = "";
-
-int fail(int code, const char *fmt, ...) {
-    va_list ap;
-    va_start(ap, fmt);
-    vsnprintf(g_err, sizeof g_err, fmt, ap);
-    va_end(ap);
-    return code;
-}
-
-int hip_check(hipError_t e, const char *what) {
-    if (e == hipSuccess) return NLS_OK;
-    return fail(e == hipErrorOutOfMemory ? NLS_ENOMEM : NLS_EHIP, "%s: %s", what, hipGetErrorString(e));
-}
-
 int check_problem(const nls_problem *p) {
     if (!p) return fail(NLS_EINVAL, "problem is NULL");
     if (p->dim != 1 && p->dim != 2) return fail(NLS_EINVAL, "dim must be 1 or 2 (got %d)", p->dim);
@@ -66,7 +51,7 @@ int check_problem(const nls_problem *p) {
 int plain_engine(const nls_problem *p) {
     if (p->engine != NLS_ENGINE_AUTO) return p->engine;
     if (resident_fits(p) && (p->dim == 1 || p->dtype == NLS_F32)) return NLS_ENGINE_RESIDENT;
-    return p->dim == 2 ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
+    return p->dim == 2 && fused_holds((long long)p->n * p->n, p->dtype) ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
 }
 
 static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
@@ -203,14 +188,6 @@ struct DevBuf {
 using namespace nls;
 
 extern "C" {
-
-const char *nls_last_error(void) { return g_err; }
-
-int nls_version(int *major, int *minor, int *patch) {
-    if (!major || !minor || !patch) return fail(NLS_EINVAL, "NULL argument");
-    *major = 0; *minor = 2; *patch = 0;
-    return NLS_OK;
-}
 
 size_t nls_workspace_bytes(const nls_problem *p) {
     if (check_problem(p)) return 0;
@@ -630,24 +607,6 @@ int nls_tune_stage_tile(int dtype, int rows) {
     if (rows == 0) return get_stage_tile(dtype);
     if (set_stage_tile(dtype, rows)) return fail(NLS_EINVAL, "tile height must be 4, 8, 16 or 32 (got %d)", rows);
     return rows;
-}
-
-int nls_make_laplacian(int n, int order, double h, double *op) {
-    if (!op) return fail(NLS_EINVAL, "NULL argument");
-    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7");
-    if (n < order) return fail(NLS_EINVAL, "n smaller than the stencil width");
-    const int R = (order - 1) / 2;
-    std::vector<double> rows((size_t)n * order);
-    for (int i = 0; i < n; ++i)
-        for (int k = 0; k < order; ++k) rows[(size_t)i * order + k] = radial_weight(i, k, n, order, h);
-    // band storage: op(p, j) = A(I, j) with I = p + j - R - 1 (1-based)
-    for (int j = 1; j <= n; ++j)
-        for (int pp = 1; pp <= order; ++pp) {
-            const int I = pp + j - R - 1;
-            op[(size_t)(j - 1) * order + (pp - 1)] =
-                (I >= 1 && I <= n) ? rows[(size_t)(I - 1) * order + (2 * R + 1 - pp)] : 0.0;
-        }
-    return NLS_OK;
 }
 
 }  // extern "C"

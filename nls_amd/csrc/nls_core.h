@@ -71,6 +71,24 @@ __device__ __forceinline__ void rhs_point_cp(const Coeffs<T> &c, T cP, cplx_t<T>
     if (DYN) kn = fma(-fma(c.d14, s, c.d13), yn, cP);
 }
 
+// The same right-hand side with the Laplacian given as its neighbour part S
+// (lap = wc y + S, wc the centre weight): the centre term joins the
+// point's own factor, q' = c5 |y|^2 + c6 r - wc, so
+//   k.re = (c3 r - c4) y.re + q' y.im - S.im,  k.im = (c3 r - c4) y.im - q' y.re + S.re,
+// which saves the two products wc y.re, wc y.im.  Equal to rhs_point_cp up to
+// rounding (q' rounds at the scale of wc, as wc y does in the sum).
+template <typename T, bool DYN>
+__device__ __forceinline__ void rhs_point_centre(const Coeffs<T> &c, T cP, T wc, cplx_t<T> y, T yn, cplx_t<T> S,
+                                                 cplx_t<T> &k, T &kn) {
+    const T s = y.x * y.x + y.y * y.y;
+    const T r = DYN ? yn : cP / (c.c13 + c.c14 * s);
+    const T g = c.c3 * r - c.c4;
+    const T q = fma(c.c5, s, fma(c.c6, r, -wc));
+    k.x = fma(g, y.x, fma(q, y.y, -S.y));
+    k.y = fma(g, y.y, fma(-q, y.x, S.x));
+    if (DYN) kn = fma(-fma(c.d14, s, c.d13), yn, cP);
+}
+
 template <typename T, bool DYN>
 __device__ __forceinline__ void rhs_point(const Coeffs<T> &c, T P, cplx_t<T> y, T yn, cplx_t<T> lap,
                                           cplx_t<T> &k, T &kn) {
@@ -110,6 +128,14 @@ __device__ __forceinline__ int wrap_index(int g, int n) {
     while (g < 0) g += n;
     while (g >= n) g -= n;
     return g;
+}
+
+// Periodic index wrap for -n <= g < 2n as one three-way unsigned minimum:
+// of g, g + n and g - n exactly one lies in [0, n) and the others, as
+// unsigned values, are larger.
+__device__ __forceinline__ int wrap_near3(int g, int n) {
+    const unsigned a = (unsigned)g, b = (unsigned)(g + n), c = (unsigned)(g - n);
+    return (int)min(min(a, b), c);
 }
 
 // Periodic index wrap for -n <= g < 2n.

@@ -1,14 +1,17 @@
 // nls_helpers.hip -- host drop-ins of the reference module's operator
-// builders and BLAS-2 wrappers: make_banded_matrix (nls/nls.f90:61-91),
+// builders and BLAS-2 wrappers: make_laplacian (nls/nls.f90:278-295),
+// make_banded_matrix (:61-91),
 // make_laplacian_2d (:297-403), rbbmv (:408-527), rgbmv (:530-541),
 // revervoir / revervoir_2d (:570-581, :829-839).  These build or apply the
 // operator arrays on the host, as the reference does; the solvers never call
 // them (their kernels hold the stencils).  Single precision, with the
 // reference's evaluation order, so results match the reference's arithmetic.
 #include <cstring>
+#include <vector>
 
 #include "nls_amd.h"
 #include "nls_internal.h"
+#include "nls_stencil.h"
 
 using namespace nls;
 
@@ -123,6 +126,26 @@ int nls_revervoir(const float *pumping, const float *coeffs, const float *u_sqr,
 int nls_revervoir_2d(const float *pumping, const float *coeffs, const float *u_sqr, float *r, int n) {
     if (n < 1) return fail(NLS_EINVAL, "bad n=%d", n);
     return reservoir(pumping, coeffs, u_sqr, r, (size_t)n * n);
+}
+
+// make_laplacian (nls.f90:278-295): the 1-D radial operator in BLAS band
+// storage, built in double precision from the closed-form row weights.
+int nls_make_laplacian(int n, int order, double h, double *op) {
+    if (!op) return fail(NLS_EINVAL, "NULL argument");
+    if (order != 3 && order != 5 && order != 7) return fail(NLS_EINVAL, "order must be 3, 5 or 7");
+    if (n < order) return fail(NLS_EINVAL, "n smaller than the stencil width");
+    const int R = (order - 1) / 2;
+    std::vector<double> rows((size_t)n * order);
+    for (int i = 0; i < n; ++i)
+        for (int k = 0; k < order; ++k) rows[(size_t)i * order + k] = radial_weight(i, k, n, order, h);
+    // band storage: op(p, j) = A(I, j) with I = p + j - R - 1 (1-based)
+    for (int j = 1; j <= n; ++j)
+        for (int pp = 1; pp <= order; ++pp) {
+            const int I = pp + j - R - 1;
+            op[(size_t)(j - 1) * order + (pp - 1)] =
+                (I >= 1 && I <= n) ? rows[(size_t)(I - 1) * order + (2 * R + 1 - pp)] : 0.0;
+        }
+    return NLS_OK;
 }
 
 }  // extern "C"

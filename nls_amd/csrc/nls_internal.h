@@ -14,6 +14,9 @@ size_t points_of(const nls_problem *p);
 
 int plain_engine(const nls_problem *p);            // RK4 engine of a plain solve (NLS_ENGINE_*)
 
+// the fused kernel holds a member (or row slab) of this many points: its
+// complex array is smaller than 4 GiB (32-bit byte offsets)
+bool fused_holds(long long points, int dtype);
 // plain 2-D RK4 solve of one grid on the fused kernel (nls_batch.hip)
 size_t fused_grid_workspace_bytes(const nls_problem *p);
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,

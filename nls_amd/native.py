@@ -37,6 +37,7 @@ EXPORTS = (
     "nls_make_banded_matrix", "nls_make_laplacian_2d", "nls_rgbmv", "nls_rbbmv", "nls_revervoir",
     "nls_revervoir_2d", "nls_hamiltonian", "nls_hamiltonian_2d", "nls_runge_kutta", "nls_runge_kutta_2d",
     "nls_batch_set_flags", "nls_batch_observables", "nls_batch_prepare", "nls_batch_energy",
+    "nls_batch_update", "nls_batch_observables_async", "nls_batch_energy_async",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2, "hybrid": 3}
 FLAG_EAGER = 1
@@ -167,6 +168,9 @@ def load():
         "nls_batch_observables": ([vp, vp, vp, vp, vp], i),
         "nls_batch_energy": ([vp, vp, vp, vp, vp, vp], i),
         "nls_batch_prepare": ([vp, i, vp, vp, vp, vp, sz], i),
+        "nls_batch_update": ([vp, C.POINTER(Member), i, vp, vp], i),
+        "nls_batch_observables_async": ([vp, vp, vp, vp, vp], i),
+        "nls_batch_energy_async": ([vp, vp, vp, vp, vp, vp], i),
         "nls_tune_stage_tile": ([i, i], i),
     }
     for name, (args, res) in sig.items():

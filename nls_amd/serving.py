@@ -11,19 +11,31 @@ schedulr/workr.py:39-107).  Here requests share the GPU:
   once at admission -- or, for requests given by parameters, generated in
   place on the device (nls_amd.fields) -- and the result downloaded once at
   completion;
-* three pools, each one batch: 1-D requests (n <= 4096) and 2-D requests up
-  to 64 x 64 on the resident engine (a workgroup per request runs every step
-  of a block in one launch), larger 2-D requests on the fused RK4 engine (one
-  launch per step); the HBM budget is split 15% / 15% / 70%;
+* three pools, each one persistent batch: 1-D requests (n <= 4096) and 2-D
+  requests up to 64 x 64 on the resident engine (a workgroup per request
+  runs every step of a block in one launch), larger 2-D requests on the
+  fused RK4 engine (one launch per step); the HBM budget is split
+  15% / 15% / 70%.  A pool's batch is created once and then updated
+  (nls_batch_update: members, budgets and tables uploaded asynchronously on
+  the pool's stream), never re-created;
 * every block advances all active requests together, each by
-  min(block, remaining) steps (nls_batch_set_steps), so a request that
-  finishes mid-block holds its state and its slot is freed at the end of the
-  block, when waiting requests are admitted; the block length adapts so one
-  block takes about target_block_ms (20 ms), or idle_block_ms (5 ms) while
-  the load is light (none waiting, at most idle_max_active in flight), so a
-  request waits less for block boundaries;
+  min(block, remaining) steps, so a request that finishes mid-block holds
+  its state; its FP64 observables, energy and state are copied to pinned
+  host memory on the pool's stream right behind the block, and its slot is
+  reused by the next admission (ordered after that copy on the same
+  stream).  The host keeps up to two blocks enqueued per pool and collects
+  a block's results once its end event has passed, so the device does not
+  wait for the host between blocks;
+* failure detection: every block computes every member's FP64 observables;
+  a request whose state holds non-finite points (a dt above the scheme's
+  stability limit) is evicted and returned with error = NLS_ENONFINITE,
+  while the others continue unchanged (the reference reports nothing,
+  nls.f90:725-733);
+* the block length adapts so one block takes about target_block_ms (20 ms),
+  or idle_block_ms (5 ms) while the load is light (none waiting, at most
+  idle_max_active in flight), so a request waits less for block boundaries;
 * each pool runs on its own stream (default priority), so the 1-D lines,
-  small grids and large grids of a block run on the device at the same time;
+  small grids and large grids run on the device at the same time;
 * a member's arithmetic never depends on the other members, so a result is
   bit-identical alone and in any mix.
 
@@ -50,6 +62,8 @@ class Request(object):
         self.rid, self.arrival, self.dim, self.n, self.steps = rid, arrival, dim, n, steps
         self.dt, self.dx, self.coeffs, self.P, self.psi0, self.n0 = dt, dx, coeffs, P, psi0, n0
         self.spec = spec
+        self.error = None               # NLS_ENONFINITE when evicted with a non-finite state
+        self.observables = None
         self.remaining = steps
         self.offset = None
         self.admit_time = self.finish_time = None
@@ -95,8 +109,23 @@ class PoolAllocator(object):
         self.free = merged
 
 
+class Block(object):
+    """One block issued on a pool: the requests it advanced, their budgets,
+    the device-side FP64 observables being copied to pinned host memory, and
+    the events that bracket it on the pool's stream."""
+
+    def __init__(self, members, budgets, finishing, start, end, obs, energy, work):
+        self.members, self.budgets, self.finishing = members, budgets, finishing
+        self.start, self.end, self.obs, self.energy, self.work = start, end, obs, energy, work
+
+
 class Group(object):
-    """Device pool and batch for the requests of one dimensionality."""
+    """Device pool and one persistent batch for the requests of one engine.
+
+    The batch is created at the first admission and then only updated
+    (nls_batch_update: members, budgets and tables uploaded asynchronously on
+    the pool's stream); blocks are enqueued without waiting for the previous
+    one, and their results are collected when their end event has passed."""
 
     def __init__(self, dim, capacity, dtype, device, engine, order=5, bc="periodic"):
         import torch
@@ -116,8 +145,10 @@ class Group(object):
         self.members = []
         self.handle = None
         self.dirty = True
+        self.inflight = []                  # Blocks enqueued and not yet collected
+        self.step_ms = None                 # measured time per step of this pool's blocks
         # the pool's own stream (default priority): the pools' launches
-        # overlap on the device; admission fills and steps are ordered on it
+        # overlap on the device; admission fills and blocks are ordered on it
         self.stream = torch.cuda.Stream(device=device)
 
     @staticmethod
@@ -127,103 +158,155 @@ class Group(object):
         return (s + r) + {"resident": 0, "fused": 1, "stage": 3}[engine] * s
 
     def admit(self, req):
+        """Place a request in the pool and enqueue its inputs on the pool's
+        stream (generated on the device, or uploaded from pinned memory)."""
         off = self.alloc.alloc(req.points)
         if off is None:
             return False
         req.offset = off
         a, b = off, off + req.points
         t = self.torch
-        ct = self.psi.dtype
-        if req.spec is not None:
-            # inputs generated in place on the device: no upload
-            sp = req.spec
-            fields.fill_profile(self.P[a:b], sp["pump"], req.dim, req.n, sp["grid"])
-            fields.fill_state(self.psi[a:b], sp["state"], sp["noise"], sp["seed"], req.dim, req.n, sp["grid"])
-            self.nres[a:b].zero_()
-        else:
-            self.psi[a:b] = t.from_numpy(np.ascontiguousarray(req.psi0.ravel())).to(self.device, ct,
-                                                                                    non_blocking=True)
-            self.P[a:b] = t.from_numpy(np.ascontiguousarray(req.P.ravel())).to(self.device, self.P.dtype,
-                                                                              non_blocking=True)
-            n0 = np.zeros(req.points) if req.n0 is None else req.n0.ravel()
-            self.nres[a:b] = t.from_numpy(np.ascontiguousarray(n0)).to(self.device, self.nres.dtype,
-                                                                        non_blocking=True)
+        with t.cuda.stream(self.stream):
+            if req.spec is not None:
+                sp = req.spec
+                fields.fill_profile(self.P[a:b], sp["pump"], req.dim, req.n, sp["grid"], stream=self.stream)
+                fields.fill_state(self.psi[a:b], sp["state"], sp["noise"], sp["seed"], req.dim, req.n, sp["grid"],
+                                  stream=self.stream)
+                self.nres[a:b].zero_()
+            else:
+                def put(dst, x):
+                    h = t.from_numpy(np.ascontiguousarray(np.asarray(x).ravel())).to(dst.dtype).pin_memory()
+                    dst.copy_(h, non_blocking=True)
+                put(self.psi[a:b], req.psi0)
+                put(self.P[a:b], req.P)
+                put(self.nres[a:b], np.zeros(req.points) if req.n0 is None else req.n0)
         self.members.append(req)
         self.dirty = True
         return True
 
-    def _rebuild(self):
-        import ctypes as C
-        if self.handle is not None:
-            native.load().nls_batch_destroy(self.handle)
-            self.handle = None
-        if not self.members:
-            return
+    def _table(self):
         arr = (native.Member * len(self.members))()
         for i, r in enumerate(self.members):
             arr[i].n, arr[i].offset, arr[i].dt, arr[i].dx = r.n, r.offset, r.dt, r.dx
             arr[i].coeffs[:] = list(np.asarray(r.coeffs, np.float64)[:23])
-        h = C.c_void_p()
-        native._check(native.load().nls_batch_create(self.dim, self.order, native.BC[self.bc],
-                                                     native.DTYPE[self.dtype], native.ENGINE[self.engine],
-                                                     arr, len(self.members), C.byref(h)))
-        self.handle = h
-        self.dirty = False
+        return arr
 
-    def advance(self, block, stream):
-        """Advance every member by min(block, remaining) steps."""
+    def issue(self, block):
+        """Enqueue one block: every member advances min(block, remaining)
+        steps; FP64 observables of every member (and the energy and result
+        download of those that finish) follow on the same stream.  Finished
+        members leave the pool at once: later fills of their slots are
+        ordered after the downloads on this stream."""
         import ctypes as C
-        if not self.members:
-            return
-        if self.dirty:
-            self._rebuild()
+        t = self.torch
         L = native.load()
-        budgets = np.array([min(block, r.remaining) for r in self.members], np.int32)
-        native._check(L.nls_batch_set_steps(self.handle, budgets.ctypes.data_as(C.c_void_p),
-                                            C.c_void_p(stream.cuda_stream)))
-        need = int(L.nls_batch_workspace_bytes(self.handle))
-        assert need <= self.ws.numel()
-        native._check(L.nls_batch_advance(self.handle, int(budgets.max()), C.c_void_p(self.P.data_ptr()),
-                                          C.c_void_p(self.psi.data_ptr()), C.c_void_p(self.nres.data_ptr()),
-                                          C.c_void_p(self.ws.data_ptr()), self.ws.numel(),
-                                          C.c_void_p(stream.cuda_stream)))
-        for r, b in zip(self.members, budgets):
+        st = C.c_void_p(self.stream.cuda_stream)
+        members = list(self.members)
+        budgets = np.array([min(block, r.remaining) for r in members], np.int32)
+        if self.handle is None:
+            h = C.c_void_p()
+            native._check(L.nls_batch_create(self.dim, self.order, native.BC[self.bc], native.DTYPE[self.dtype],
+                                             native.ENGINE[self.engine], self._table(), len(members), C.byref(h)))
+            self.handle = h
+            native._check(L.nls_batch_set_flags(h, native.FLAG_EAGER))
+            self.dirty = True
+        if self.dirty:
+            native._check(L.nls_batch_update(self.handle, self._table(), len(members),
+                                             budgets.ctypes.data_as(C.c_void_p), st))
+            self.dirty = False
+        else:
+            native._check(L.nls_batch_set_steps(self.handle, budgets.ctypes.data_as(C.c_void_p), st))
+        assert int(L.nls_batch_workspace_bytes(self.handle)) <= self.ws.numel()
+        start, end = t.cuda.Event(enable_timing=True), t.cuda.Event(enable_timing=True)
+        finishing = [r for r, b in zip(members, budgets) if int(b) == r.remaining]
+        with t.cuda.stream(self.stream):
+            start.record(self.stream)
+            native._check(L.nls_batch_advance(self.handle, int(budgets.max()), C.c_void_p(self.P.data_ptr()),
+                                              C.c_void_p(self.psi.data_ptr()), C.c_void_p(self.nres.data_ptr()),
+                                              C.c_void_p(self.ws.data_ptr()), self.ws.numel(), st))
+            obs_d = t.empty(4 * len(members), dtype=t.float64, device=self.device)
+            native._check(L.nls_batch_observables_async(self.handle, C.c_void_p(self.psi.data_ptr()),
+                                                        C.c_void_p(self.nres.data_ptr()),
+                                                        C.c_void_p(obs_d.data_ptr()), st))
+            obs = t.empty(obs_d.shape, dtype=t.float64, pin_memory=True)
+            obs.copy_(obs_d, non_blocking=True)
+            energy = None
+            if finishing:
+                en_d = t.empty(4 * len(members), dtype=t.float64, device=self.device)
+                native._check(L.nls_batch_energy_async(self.handle, C.c_void_p(self.P.data_ptr()),
+                                                       C.c_void_p(self.psi.data_ptr()),
+                                                       C.c_void_p(self.nres.data_ptr()),
+                                                       C.c_void_p(en_d.data_ptr()), st))
+                energy = t.empty(en_d.shape, dtype=t.float64, pin_memory=True)
+                energy.copy_(en_d, non_blocking=True)
+                for r in finishing:
+                    a, b = r.offset, r.offset + r.points
+                    r._psi_host = t.empty(r.points, dtype=self.psi.dtype, pin_memory=True)
+                    r._nres_host = t.empty(r.points, dtype=self.nres.dtype, pin_memory=True)
+                    r._psi_host.copy_(self.psi[a:b], non_blocking=True)
+                    r._nres_host.copy_(self.nres[a:b], non_blocking=True)
+            end.record(self.stream)
+        work = 0
+        for r, b in zip(members, budgets):
             r.remaining -= int(b)
-
-    def retire(self, now_fn):
-        """Download and free the members that have finished, with their FP64
-        observables (norm, population, peak density, non-finite count,
-        energy)."""
-        import ctypes as C
-        done = [r for r in self.members if r.remaining == 0]
-        if not done:
-            return []
-        obs = np.zeros((len(self.members), 4), np.float64)      # one launch for the whole pool
-        native._check(native.load().nls_batch_observables(
-            self.handle, C.c_void_p(self.psi.data_ptr()), C.c_void_p(self.nres.data_ptr()),
-            obs.ctypes.data_as(C.c_void_p), C.c_void_p(self.stream.cuda_stream)))
-        en = np.zeros((len(self.members), 4), np.float64)
-        native._check(native.load().nls_batch_energy(
-            self.handle, C.c_void_p(self.P.data_ptr()), C.c_void_p(self.psi.data_ptr()),
-            C.c_void_p(self.nres.data_ptr()), en.ctypes.data_as(C.c_void_p), C.c_void_p(self.stream.cuda_stream)))
-        index = {id(r): i for i, r in enumerate(self.members)}
-        for r in done:
-            o = obs[index[id(r)]]
-            r.observables = {"norm": float(o[0]), "population": float(o[1]), "peak": float(o[2]),
-                             "nonfinite": int(o[3]), "energy": float(en[index[id(r)]][0])}
-        for r in done:
-            a, b = r.offset, r.offset + r.points
-            shape = (r.n,) if r.dim == 1 else (r.n, r.n)
-            r.psi = self.psi[a:b].cpu().numpy().reshape(shape)
-            r.nres = self.nres[a:b].cpu().numpy().reshape(shape)
-            r.finish_time = now_fn()
+            work += r.points * int(b)
+        for r in finishing:
             self.alloc.release(r.offset, r.points)
             self.members.remove(r)
-        self.dirty = True
-        return done
+            self.dirty = True
+        blk = Block(members, budgets, finishing, start, end, obs, energy, work)
+        self.inflight.append(blk)
+        return blk
+
+    def collect(self, now_fn, wait=False):
+        """Results of the blocks whose end event has passed (in issue order):
+        finished requests get their state, FP64 observables and energy; a
+        request whose state holds non-finite points is evicted with
+        error = NLS_ENONFINITE.  Returns (requests done, blocks collected)."""
+        done, blocks = [], []
+        while self.inflight and (wait or self.inflight[0].end.query()):
+            blk = self.inflight.pop(0)
+            blk.end.synchronize()
+            wait = False
+            obs = blk.obs.numpy().reshape(-1, 4)
+            en = blk.energy.numpy().reshape(-1, 4) if blk.energy is not None else None
+            ms = blk.start.elapsed_time(blk.end)
+            steps = int(blk.budgets.max()) if len(blk.budgets) else 0
+            if steps:
+                per = ms / steps
+                self.step_ms = per if self.step_ms is None else 0.5 * (self.step_ms + per)
+            fin = set(id(r) for r in blk.finishing)
+            for i, r in enumerate(blk.members):
+                bad = obs[i, 3] > 0
+                if id(r) in fin:
+                    r.observables = {"norm": float(obs[i, 0]), "population": float(obs[i, 1]),
+                                     "peak": float(obs[i, 2]), "nonfinite": int(obs[i, 3]),
+                                     "energy": float(en[i, 0])}
+                    shape = (r.n,) if r.dim == 1 else (r.n, r.n)
+                    r.psi = r._psi_host.numpy().reshape(shape)
+                    r.nres = r._nres_host.numpy().reshape(shape)
+                    r._psi_host = r._nres_host = None
+                    if bad:
+                        r.error = native.NLS_ENONFINITE
+                    r.finish_time = now_fn()
+                    done.append(r)
+                elif bad and r.error is None and r in self.members:
+                    # blown up (e.g. dt above the scheme's stability limit):
+                    # evicted; the blocks already enqueued only touch its slot
+                    r.error = native.NLS_ENONFINITE
+                    r.observables = {"norm": float(obs[i, 0]), "population": float(obs[i, 1]),
+                                     "peak": float(obs[i, 2]), "nonfinite": int(obs[i, 3]), "energy": None}
+                    self.alloc.release(r.offset, r.points)
+                    self.members.remove(r)
+                    self.dirty = True
+                    r.finish_time = now_fn()
+                    done.append(r)
+            blocks.append(blk)
+        return done, blocks
 
     def close(self):
         if self.handle is not None:
+            self.stream.synchronize()
             native.load().nls_batch_destroy(self.handle)
             self.handle = None
 
@@ -235,12 +318,14 @@ class Engine(object):
     """Continuous batching under an HBM budget (bytes).  Three pools: 1-D
     requests and 2-D requests up to 64^2 on the resident engine (a
     workgroup per request, every step of a block in one launch), larger 2-D
-    requests on the fused engine (one launch per step)."""
+    requests on the fused engine (one launch per step).  Each pool keeps up
+    to `inflight` blocks enqueued, so the device never waits for the host
+    between blocks."""
 
     SHARES = {"line": 0.15, "small": 0.15, "large": 0.70}
 
     def __init__(self, budget_bytes=8 << 30, dtype="f32", device=None, target_block_ms=20.0,
-                 min_block=8, max_block=2000, idle_block_ms=5.0, idle_max_active=32):
+                 min_block=8, max_block=2000, idle_block_ms=5.0, idle_max_active=32, inflight=2):
         import torch
         self.torch = torch
         self.device = device or torch.device("cuda:0")
@@ -256,53 +341,63 @@ class Engine(object):
         # admitted or finished mid-block waits less for the block's end
         self.idle_block_ms = target_block_ms if idle_block_ms is None else idle_block_ms
         self.idle_max_active = idle_max_active
-        self.block = min_block
+        self.inflight = inflight
         self.blocks = 0
+        self.block_log = []          # (host time the block was collected, grid-point-steps it advanced)
+
+    def _block_len(self, g, light):
+        target = self.idle_block_ms if light else self.target_block_ms
+        if g.step_ms is None:
+            return self.min_block
+        return int(min(self.max_block, max(self.min_block, target / max(g.step_ms, 1e-4))))
 
     def run(self, requests, clock=None, log=None):
-        """Serve `requests` (sorted by arrival, seconds from start).  Returns them
-        with admit/finish times filled in."""
+        """Serve `requests` (sorted by arrival, seconds from start).  Returns
+        them as they finish, with admit/finish times (and, for a request
+        evicted because its state went non-finite, error = NLS_ENONFINITE)."""
         t0 = time.perf_counter()
         now = clock or (lambda: time.perf_counter() - t0)
         pending = list(requests)
+        for r in pending:
+            r.error = None
+        pi = 0
         waiting = []
         finished = []
-        while pending or waiting or any(g.members for g in self.groups.values()):
+        groups = list(self.groups.values())
+        while pi < len(pending) or waiting or any(g.members or g.inflight for g in groups):
             t = now()
-            while pending and pending[0].arrival <= t:
-                waiting.append(pending.pop(0))
+            while pi < len(pending) and pending[pi].arrival <= t:
+                waiting.append(pending[pi])
+                pi += 1
+            for g in groups:
+                done, blocks = g.collect(now)
+                finished += done
+                for blk in blocks:
+                    self.block_log.append((now(), blk.work))
             still = []
             for r in waiting:                       # arrival order; later ones may pass
                 g = self.groups[self.group_of(r)]
-                with self.torch.cuda.stream(g.stream):
-                    ok = g.admit(r)
-                if ok:
+                if g.admit(r):
                     r.admit_time = now()
                 else:
                     still.append(r)
             waiting = still
-            active = [g for g in self.groups.values() if g.members]
-            if not active:
-                if pending:
-                    time.sleep(max(0.0, min(pending[0].arrival - now(), 0.002)))
+            light = not waiting and sum(len(g.members) for g in groups) <= self.idle_max_active
+            issued = False
+            for g in groups:
+                if g.members and len(g.inflight) < self.inflight:
+                    g.issue(self._block_len(g, light))
+                    self.blocks += 1
+                    issued = True
+            if issued:
                 continue
-            tb = time.perf_counter()
-            for g in active:                        # each pool on its own stream, concurrently
-                g.advance(self.block, g.stream)
-            for g in active:
-                g.stream.synchronize()
-            for g in active:
-                finished += g.retire(now)
-            self.blocks += 1
-            el = (time.perf_counter() - tb) * 1e3
-            per_step = el / max(self.block, 1)
-            light = not waiting and sum(len(g.members) for g in active) <= self.idle_max_active
-            target = self.idle_block_ms if light else self.target_block_ms
-            self.block = int(min(self.max_block, max(self.min_block, target / max(per_step, 1e-4))))
+            if any(g.inflight for g in groups):
+                time.sleep(0.0002)                  # every pool has its blocks queued: poll again
+            elif pi < len(pending):
+                time.sleep(max(0.0, min(pending[pi].arrival - now(), 0.002)))
             if log and self.blocks % 50 == 0:
-                log("t=%.2fs finished=%d waiting=%d active=%d block=%d" % (
-                    now(), len(finished), len(waiting), sum(len(g.members) for g in self.groups.values()),
-                    self.block))
+                log("t=%.2fs finished=%d waiting=%d active=%d" % (
+                    now(), len(finished), len(waiting), sum(len(g.members) for g in groups)))
         return finished
 
     @staticmethod
@@ -383,29 +478,69 @@ def poisson_requests(rate, duration, seed=0, step_range=(1e3, 1e5), device_input
     return out
 
 
-def summarize(done, wall):
-    lat = np.array([r.latency for r in done])
-    work = sum(r.points * r.steps for r in done)
-    return {"requests": len(done), "p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
-            "mean_s": float(lat.mean()), "max_s": float(lat.max()), "throughput_req_s": len(done) / wall,
-            "grid_point_steps_per_s": work / wall}
+def summarize(done, wall, engine=None, window=None, offered=None):
+    """Latency percentiles and throughput of a served run.  With the engine's
+    block log and the arrival window W: the steady-state rate is the work of
+    the blocks collected during [0.2 W, W] (arrivals still coming, the
+    first fifth left out as ramp-up) per second, without the drain after the
+    last arrival."""
+    ok = [r for r in done if r.error is None]
+    lat = np.array([r.latency for r in ok])
+    work = sum(r.points * r.steps for r in ok)
+    out = {"requests": len(done), "failed": len(done) - len(ok), "p50_s": float(np.percentile(lat, 50)),
+           "p99_s": float(np.percentile(lat, 99)), "mean_s": float(lat.mean()), "max_s": float(lat.max()),
+           "throughput_req_s": len(ok) / wall, "grid_point_steps_per_s": work / wall}
+    if engine is not None and window:
+        a, b = 0.2 * window, window
+        steady = sum(w for t, w in engine.block_log if a <= t <= b)
+        out["steady_grid_point_steps_per_s"] = steady / (b - a)
+        out["steady_window_s"] = [a, b]
+    if offered is not None and window:
+        out["offered_grid_point_steps_per_s"] = offered / window
+    return out
 
 
-def sweep(rates=(20, 50, 100, 200, 500), duration=4.0, budget_gb=16, seed=0, step_range=(1e3, 1e5), log=print):
+def mean_request_work(samples=20000, seed=12345, step_range=(1e3, 1e5)):
+    """Mean grid-point-steps of a request of the serving mix (sampled)."""
+    rng = np.random.default_rng(seed)
+    tot = 0.0
+    for _ in range(samples):
+        dim = 1 if rng.random() < 0.5 else 2
+        lo, hi = (256, 4096) if dim == 1 else (64, 512)
+        n = int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))) / 64.0)) * 64
+        steps = int(round(math.exp(rng.uniform(math.log(step_range[0]), math.log(step_range[1])))))
+        tot += (n if dim == 1 else n * n) * steps
+    return tot / samples
+
+
+def window_for(rate, min_requests=200, min_window=1.0):
+    """Arrival window (s) holding about min_requests requests at `rate`."""
+    return max(min_window, min_requests / float(rate))
+
+
+def serve(rate, window=None, budget_gb=16, seed=0, step_range=(1e3, 1e5), device_inputs=True):
+    """One run of the request loop: Poisson arrivals at `rate` over `window`
+    seconds (default: about 200 requests), the serving mix; returns the
+    summary."""
     import torch
+    window = window or window_for(rate)
+    eng = Engine(budget_bytes=int(budget_gb * (1 << 30)))
+    reqs = poisson_requests(rate, window, seed=seed, step_range=step_range, device_inputs=device_inputs)
+    offered = sum(r.points * r.steps for r in reqs)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = eng.run(reqs)
+    wall = time.perf_counter() - t0
+    eng.close()
+    s = summarize(done, wall, engine=eng, window=window, offered=offered)
+    s.update({"rate_per_s": rate, "arrival_window_s": window, "wall_s": wall, "blocks": eng.blocks})
+    return s
+
+
+def sweep(rates=(20, 50, 100, 200, 500), budget_gb=16, seed=0, step_range=(1e3, 1e5), log=print):
     out = []
     for lam in rates:
-        eng = Engine(budget_bytes=int(budget_gb * (1 << 30)))
-        reqs = poisson_requests(lam, duration, seed=seed, step_range=step_range)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        done = eng.run(reqs)
-        wall = time.perf_counter() - t0
-        s = summarize(done, wall)
-        s["rate"] = lam
-        s["arrival_window_s"] = duration
-        s["wall_s"] = wall
-        eng.close()
+        s = serve(lam, budget_gb=budget_gb, seed=seed, step_range=step_range)
         if log:
             log(s)
         out.append(s)

@@ -81,3 +81,79 @@ def test_result_alone_equals_result_in_the_mix(gpu, oracle):
         assert np.isfinite(r.observables["energy"])
         assert r.observables["norm"] == pytest.approx(s2.sum() * dV, rel=1e-12)
         assert r.observables["population"] == pytest.approx(r.nres.astype(np.float64).sum() * dV, rel=1e-12)
+
+
+def _request_2d(rid, n, steps, seed, dt=None, length=32.0, arrival=0.0):
+    from nls_amd import configs
+    rng = np.random.default_rng(seed)
+    X, Y, h = configs.grid_2d(n, length)
+    r = np.sqrt(X ** 2 + Y ** 2)
+    P = configs.ring(r, rng.uniform(1.0, 4.0), rng.uniform(3.0, 6.0), rng.uniform(0.5, 1.5))
+    psi0 = 0.1 * np.exp(-r ** 2 / 4.0) + configs.noise(rng, r.shape)
+    return serving.Request(rid, arrival, 2, n, steps, serving.stable_dt(2, h) if dt is None else dt, h,
+                           configs.C_MODEL.copy(), P, psi0, np.zeros(r.shape))
+
+
+def _request_1d(rid, n, steps, seed, arrival=0.0):
+    from nls_amd import configs
+    rng = np.random.default_rng(seed)
+    x, h = configs.grid_1d(n, 64.0)
+    P = rng.uniform(1.0, 4.0) * np.exp(-x ** 2 / (2 * rng.uniform(1.0, 3.0) ** 2))
+    psi0 = 0.1 * np.exp(-x ** 2 / 4.0) + configs.noise(rng, x.shape)
+    return serving.Request(rid, arrival, 1, n, steps, serving.stable_dt(1, h), h, configs.C_MODEL.copy(), P, psi0,
+                           np.zeros(x.shape))
+
+
+@pytest.mark.gpu
+def test_large_grid_and_long_request_match_oracle(gpu, oracle):
+    """The request loop in double precision with the sizes of the serving
+    mix's large pool: a 512 x 512 request (fused pool) beside smaller ones,
+    and a 1-D request of 1300 steps, both with budgets ending inside a block
+    (blocks of at most 256 steps); each result against the oracle's solve of
+    the same request (relative L2 <= 1e-10)."""
+    import os
+    reqs = [_request_2d(0, 512, 300, seed=1), _request_1d(1, 1024, 1300, seed=2),
+            _request_2d(2, 192, 700, seed=3, arrival=0.01), _request_2d(3, 64, 500, seed=4)]
+    eng = serving.Engine(budget_bytes=2 << 30, dtype="f64", target_block_ms=2.0, min_block=8, max_block=256)
+    done = eng.run(reqs)
+    eng.close()
+    assert len(done) == len(reqs) and all(r.error is None for r in done)
+    assert eng.blocks > 6
+    oracle.set_threads(min(16, os.cpu_count() or 1))
+    for r in done:
+        if r.rid not in (0, 1):
+            continue
+        opsi, onres = oracle.rk4(r.dim, r.n, 5, 1, r.dt, r.dx, r.steps, r.P, r.coeffs, r.psi0, r.n0)
+        err = np.linalg.norm(r.psi - opsi) / np.linalg.norm(opsi)
+        assert err <= 1e-10, (r.rid, err)
+        assert np.linalg.norm(r.nres - onres) / np.linalg.norm(onres) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_blown_up_request_is_evicted_and_others_unchanged(gpu):
+    """A request whose dt is far above the RK4 stability limit blows up; the
+    loop evicts it once its FP64 observables count non-finite points and
+    returns it with error NLS_ENONFINITE, while the healthy requests beside it
+    in the same pool finish bit for bit as when served without it."""
+    from nls_amd import native
+
+    def healthy():
+        return [_request_2d(0, 128, 600, seed=11), _request_2d(1, 96, 400, seed=12), _request_1d(2, 512, 500, seed=13)]
+
+    bad = _request_2d(9, 128, 5000, seed=14)
+    bad.dt = 40.0 * bad.dt
+    kw = dict(budget_bytes=1 << 30, target_block_ms=2.0, min_block=8, max_block=64)
+    eng = serving.Engine(**kw)
+    done = {r.rid: r for r in eng.run(healthy() + [bad])}
+    eng.close()
+    ref_eng = serving.Engine(**kw)
+    ref = {r.rid: r for r in ref_eng.run(healthy())}
+    ref_eng.close()
+    b = done[9]
+    assert b.error == native.NLS_ENONFINITE and b.psi is None
+    assert b.observables["nonfinite"] > 0
+    assert b.remaining > 0                      # evicted before its steps ran out
+    for rid in (0, 1, 2):
+        assert done[rid].error is None
+        np.testing.assert_array_equal(done[rid].psi, ref[rid].psi)
+        np.testing.assert_array_equal(done[rid].nres, ref[rid].nres)
