@@ -120,6 +120,12 @@ int nls_make_laplacian(int n, int order, double h, double *op);
  *   revervoir / revervoir_2d (:570-581, :829-839):
  *     r = c12 P / (c13 + c14 u_sqr) on n or n*n points. */
 int nls_make_banded_matrix(int n, int m, const float *row, float *mat);
+/* clear_first_row_of_derivative (:93-107), divide_derivative_on_radius
+ * (:109-130): in place on a first-derivative band L1(m, n) -- the steps
+ * make_laplacian_o3/o5/o7 take: zero the entries of matrix row 0, divide the
+ * entries of matrix row k > 0 by k h. */
+int nls_clear_first_row_of_derivative(int n, int m, float *L1);
+int nls_divide_derivative_on_radius(int n, int m, float h, float *L1);
 int nls_make_laplacian_2d(int n, int m, float h, float *blocks, int *orders);
 int nls_rgbmv(const float *x, float *u, float sign, const float *op, int klu, int n);
 int nls_rbbmv(const float *x, float *y, float sign, const float *blocks, const int *ms, int m, int n);

@@ -148,4 +148,28 @@ int nls_make_laplacian(int n, int order, double h, double *op) {
     return NLS_OK;
 }
 
+// clear_first_row_of_derivative (nls.f90:93-107) and
+// divide_derivative_on_radius (:109-130), in place on a first-derivative band
+// L1(m, n) in BLAS band storage (column-major, L1(i, j) at L1[(j-1) m + i-1]),
+// as make_laplacian_o3/o5/o7 apply them: zero the band entries of the first
+// (m+1)/2 entries that lie on matrix row 0, then divide every entry on matrix
+// row k = i + j - (m+3)/2 > 0 by r_k = k h, in single precision.
+int nls_clear_first_row_of_derivative(int n, int m, float *L1) {
+    if (!L1) return fail(NLS_EINVAL, "NULL argument");
+    if (m < 1 || m % 2 == 0 || n < (m + 1) / 2) return fail(NLS_EINVAL, "bad band shape (m=%d, n=%d)", m, n);
+    for (int i = 1; i <= (m + 1) / 2; ++i) L1[(size_t)((m + 1) / 2 - i) * m + (i - 1)] = 0.0f;
+    return NLS_OK;
+}
+
+int nls_divide_derivative_on_radius(int n, int m, float h, float *L1) {
+    if (!L1) return fail(NLS_EINVAL, "NULL argument");
+    if (m < 1 || m % 2 == 0 || n < 1) return fail(NLS_EINVAL, "bad band shape (m=%d, n=%d)", m, n);
+    for (int i = 1; i <= m; ++i)
+        for (int j = 1; j <= n; ++j) {
+            const int row = i + j - (m + 3) / 2;
+            if (row > 0) L1[(size_t)(j - 1) * m + (i - 1)] = L1[(size_t)(j - 1) * m + (i - 1)] / ((float)row * h);
+        }
+    return NLS_OK;
+}
+
 }  // extern "C"

@@ -38,6 +38,7 @@ EXPORTS = (
     "nls_revervoir_2d", "nls_hamiltonian", "nls_hamiltonian_2d", "nls_runge_kutta", "nls_runge_kutta_2d",
     "nls_batch_set_flags", "nls_batch_observables", "nls_batch_prepare", "nls_batch_energy",
     "nls_batch_update", "nls_batch_observables_async", "nls_batch_energy_async",
+    "nls_clear_first_row_of_derivative", "nls_divide_derivative_on_radius",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2, "hybrid": 3}
 FLAG_EAGER = 1
@@ -141,6 +142,8 @@ def load():
         "nls_release": ([], i),
         "nls_slab_halo_rows": ([i], i),
         "nls_make_banded_matrix": ([i, i, vp, vp], i),
+        "nls_clear_first_row_of_derivative": ([i, i, vp], i),
+        "nls_divide_derivative_on_radius": ([i, i, f, vp], i),
         "nls_make_laplacian_2d": ([i, i, f, vp, vp], i),
         "nls_rgbmv": ([vp, vp, f, vp, i, i], i),
         "nls_rbbmv": ([vp, vp, f, vp, vp, i, i], i),
@@ -279,9 +282,38 @@ class _NlsModule(object):
     # follow f2py's rules: intent(out) arrays are returned, intent(inout)
     # arrays (rgbmv's u, rbbmv's y) are updated in place and must be float32
     # contiguous arrays, dimension arguments that an input array's shape
-    # fixes are optional and come last.  clear_first_row_of_derivative and
-    # divide_derivative_on_radius read their intent(out) array before writing
-    # it (nls.f90:93-130): there is no meaningful binding, and none is made.
+    # fixes are optional and come last.
+
+    @staticmethod
+    def _band_inplace(L1, n, m):
+        """The band L1(m, n) as a Fortran-ordered float32 array: the caller's,
+        updated in place when it already is one; a new one otherwise."""
+        if L1 is None:
+            # f2py allocates an intent(out) array afresh, zero-filled: the
+            # procedure then works on zeros (parity unpinned: no build of the
+            # reference's extension here)
+            return np.zeros((int(m), int(n)), np.float32, order="F")
+        a = L1 if (isinstance(L1, np.ndarray) and L1.dtype == np.float32 and L1.flags.f_contiguous) else \
+            _arr(L1, np.float32, "l1", "3rd", None, order="F")
+        if a.shape != (int(m), int(n)):
+            raise ValueError("l1 has shape %s, expected %s" % (a.shape, (int(m), int(n))))
+        return a
+
+    def clear_first_row_of_derivative(self, n, m, L1=None):
+        """l1 = clear_first_row_of_derivative(n, m[, l1]) -- nls.f90:93-107.  The
+        Fortran procedure works in place on make_laplacian's first-derivative
+        band; given a Fortran-ordered float32 l1 of shape (m, n) it does so here
+        too, and returns it."""
+        a = self._band_inplace(L1, n, m)
+        _check(load().nls_clear_first_row_of_derivative(int(n), int(m), _ptr(a)))
+        return a
+
+    def divide_derivative_on_radius(self, n, m, h, L1=None):
+        """l1 = divide_derivative_on_radius(n, m, h[, l1]) -- nls.f90:109-130, in
+        place on l1 as clear_first_row_of_derivative."""
+        a = self._band_inplace(L1, n, m)
+        _check(load().nls_divide_derivative_on_radius(int(n), int(m), float(h), _ptr(a)))
+        return a
 
     def make_laplacian_o3(self, n, h):
         return self.make_laplacian(n, 3, h)

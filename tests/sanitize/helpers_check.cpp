@@ -27,6 +27,9 @@ int main() {
             std::vector<float> row(m), mat((size_t)m * n);
             for (int k = 0; k < m; ++k) row[k] = (float)(k + 1);
             if (nls_make_banded_matrix(n, m, row.data(), mat.data())) bad = 1;
+            if (nls_clear_first_row_of_derivative(n, m, mat.data())) bad = 1;
+            if (nls_divide_derivative_on_radius(n, m, 0.1f, mat.data())) bad = 1;
+            finite("derivative band", mat);
             std::vector<float> blocks((size_t)n * (2 * m - 1)), x((size_t)n * n, 1.0f), y((size_t)n * n, 0.0f);
             std::vector<int> orders(m);
             if (nls_make_laplacian_2d(n, m, 0.1f, blocks.data(), orders.data())) bad = 1;

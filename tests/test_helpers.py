@@ -104,3 +104,25 @@ def test_operator_arguments_checked_on_the_host():
         nls.runge_kutta_2d(1e-3, 0.0, np.ones((n, n)), bad, orders, 1, np.ones((n, n)), c)
     with pytest.raises(ValueError):
         nls.hamiltonian_2d(np.ones((n, n)), c, np.ones((n, n)), blocks, np.array([0, 1, 1, 1, 0]))
+
+
+@pytest.mark.parametrize("m", (3, 5, 7))
+def test_derivative_band_steps(m):
+    """clear_first_row_of_derivative and divide_derivative_on_radius
+    (nls.f90:93-130) in place on a band of ones, against the closed form:
+    band entry (i, j) lies on matrix row k = i + j - (m+3)/2; row 0 becomes
+    0, rows k > 0 become 1/(k h) (one single-precision division), entries
+    outside the matrix (k < 0) are left alone."""
+    from nls_amd.native import nls
+    n, h = 12, np.float32(0.3)
+    L1 = np.ones((m, n), np.float32, order="F")
+    out = nls.clear_first_row_of_derivative(n, m, L1)
+    assert out is L1
+    nls.divide_derivative_on_radius(n, m, float(h), L1)
+    for i in range(1, m + 1):
+        for j in range(1, n + 1):
+            k = i + j - (m + 3) // 2
+            want = np.float32(1.0) if k < 0 else (np.float32(0.0) if k == 0 else np.float32(1.0) / (np.float32(k) * h))
+            assert L1[i - 1, j - 1] == want, (i, j, k)
+    # without a band: f2py's fresh zero-filled intent(out) array
+    assert not nls.clear_first_row_of_derivative(n, m).any()
