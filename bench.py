@@ -466,6 +466,11 @@ def main():
         engine = tuner.batch_engine(members, dim=2, order=5, bc="periodic", dtype="f32")
         tuning = {k: v * 1e6 for k, v in tuner.last.items()}
         log("tuned batch engine: %s (%s us/step)" % (engine, ", ".join("%s %.1f" % kv for kv in tuning.items())))
+        if engine in ("fused", "hybrid"):
+            form = tuner.fused_form(members, dim=2, order=5, bc="periodic", dtype="f32", engine=engine)
+            tuning = {"engine": tuning, "fused_form": {k: v * 1e6 for k, v in tuner.last.items()}}
+            log("tuned fused tile form: %s (%s us/step)" % (form, ", ".join(
+                "%s %.1f" % kv for kv in tuning["fused_form"].items())))
     batch = Batch(members, dim=2, order=5, bc="periodic", dtype="f32", engine=engine, device=dev)
     stream = torch.cuda.current_stream()
     if W > 0:
@@ -537,6 +542,7 @@ def main():
                                   "order 5, dynamic reservoir, dt=5e-4, %d RK4 steps, complex64 state" % K,
                       "instances": len(members), "grid_points": points, "engine": engine,
                       "step_blocks": blocks, "graphs_replayed": bool(graphs and engine != "resident"),
+                      "fused_form": native.fused_form("f32"),
                       "parallelism": "single GPU"},
            "roofline": roofline}
     if tuning is not None:
@@ -552,6 +558,7 @@ def main():
         obs.shape[0], out["fp64_observables"]["nonfinite_members"], obs[:, 0].min(), obs[:, 0].max(),
         en[:, 0].min(), en[:, 0].max()))
     batch.close()
+    native.fused_form("f32", "auto")         # the other legs take the per-launch rule
 
     if not args.no_plain:
         # the headline's qualifier at the workload's full step count, whatever

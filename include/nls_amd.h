@@ -227,6 +227,16 @@ int nls_rhs(const nls_problem *p, const void *pumping, const void *psi, const vo
  * counterpart in the reference. */
 int nls_tune_stage_tile(int dtype, int rows);
 
+/* Tile form of the fused RK4 kernel for dtype: 0 chooses it per launch from
+ * the tile counts against the compute units (the default), 1 = 64 x 32 tiles
+ * of 1024 threads (64 x 16 at order 7), 2 = 64 x 16 tiles of 1024 threads,
+ * 3 = 64 x 16 tiles of 512 threads with one stage buffer (double precision
+ * on periodic grids; elsewhere it runs as 1); form = -1 only queries.
+ * Returns the setting in effect or a negative NLS_E* code.  Process-wide; a
+ * batch takes the form when it is created or updated.  Set by the
+ * autotuner.  No counterpart in the reference. */
+int nls_tune_fused_form(int dtype, int form);
+
 /* Chemical potential of a state on device, mu = i <psi, F(psi)>_w / <psi, psi>_w
  * with the weight r_k = k*dx on the 1-D radial grid and 1 otherwise
  * (chemical_potential_1d/2d, nls.f90:921-971, generalised to the problem's

@@ -13,7 +13,14 @@ What is tuned:
   * ``stage_tile(dtype)``    -- tile height (4, 8, 16, 32 rows) of the 2-D
     per-stage kernels (nls_tune_stage_tile), applied to the library;
   * ``batch_engine(members, ...)`` -- engine of a batch (nls_batch_create):
-    resident, fused or per-stage for that member mix.
+    resident, fused or per-stage for that member mix;
+  * ``fused_form(members, ...)`` / ``plain_fused_form(desc)`` -- tile shape
+    of the fused kernel (nls_tune_fused_form), applied to the library:
+    64 x 32 tiles of 1024 threads ("tall"), 64 x 16 tiles of 1024 threads
+    ("short"), or, in double precision on periodic grids, 64 x 16 tiles of 512
+    threads with one stage buffer ("half").  The halo is not a free
+    parameter: one launch per RK4 step needs the 4R rings its four stages
+    consume (R the stencil radius).
 
 The cache file is ``$NLS_AMD_TUNE_CACHE`` or ``~/.cache/nls_amd/tune.json``;
 ``Tuner(path=None)`` keeps the results in memory only.
@@ -157,6 +164,57 @@ class Tuner(object):
         if self._timer is None:
             native.stage_tile(dtype, best if apply else prev)
         return best
+
+    # ---- fused kernel tile form ---------------------------------------------
+    @staticmethod
+    def form_candidates(dtype, bc, order):
+        if order == 7:
+            return ["tall"]                        # one form at order 7
+        return ["tall", "short"] + (["half"] if dtype in ("f64", 1) and bc in ("periodic", 1) else [])
+
+    def _form_choice(self, key, dtype, cands, run, apply):
+        prev = native.fused_form(dtype) if self._timer is None else None
+
+        def timed(form):
+            if self._timer is None:
+                native.fused_form(dtype, form)
+            return run(form)
+
+        best = self._choose(key, cands, timed)
+        if self._timer is None:
+            native.fused_form(dtype, best if apply else prev)
+        return best
+
+    def fused_form(self, members, dim=2, order=5, bc="periodic", dtype="f32", engine="fused", apply=True):
+        """Fastest tile form of the fused kernel for a batch of this member
+        mix on `engine` ("fused" or "hybrid"); applied to the library."""
+        sizes = {}
+        for m in members:
+            sizes[int(m.n)] = sizes.get(int(m.n), 0) + 1
+        mix = ",".join("%dx%d" % (n, c) for n, c in sorted(sizes.items()))
+        dyn = int(np.asarray(members[0].coeffs)[10] != 0.0)
+        key = self._key("fused_form", dim=dim, order=order, bc=bc, dtype=dtype, dyn=dyn, mix=mix, engine=engine)
+
+        def run(form):
+            if self._timer is not None:
+                return self._timer(("fused_form", form), members)
+            return self._time_batch(members, dim, order, bc, dtype, engine)
+
+        return self._form_choice(key, dtype, self.form_candidates(dtype, bc, order), run, apply)
+
+    def plain_fused_form(self, desc, apply=True):
+        """Fastest tile form of the fused kernel for a plain 2-D solve of
+        desc's problem class; applied to the library."""
+        dtype = "f64" if desc.dtype == 1 else "f32"
+        bc = "periodic" if desc.bc == 1 else "ref"
+        key = self._key("plain_fused_form", n=desc.n, dtype=dtype, order=desc.order, bc=bc, dyn=int(desc.dynamic))
+
+        def run(form):
+            if self._timer is not None:
+                return self._timer(("plain_fused_form", form), desc)
+            return self._time_plain(desc, "fused")
+
+        return self._form_choice(key, dtype, self.form_candidates(dtype, bc, desc.order), run, apply)
 
     # ---- batches -----------------------------------------------------------
     def batch_engine(self, members, dim=2, order=5, bc="periodic", dtype="f32"):

@@ -809,7 +809,24 @@ static int device_cus() {
 //    196 -> 190 us; at two rounds or fewer the tall tiles are as fast or
 //    faster (1024^2: 41 us tall, 43 us half; a 2048 x 256 slab, one round:
 //    13.1 us tall, 14.9 us half).
+// A process-wide choice of form per dtype (nls_tune_fused_form, set by the
+// autotuner), -1: the rule below.
+static std::atomic<int> g_fused_form[2] = {{-1}, {-1}};
+
+int set_fused_form(int dtype, int form) {
+    if (form < -1 || form > FORM_HALF) return -1;
+    g_fused_form[dtype == NLS_F64 ? 1 : 0].store(form);
+    return 0;
+}
+int get_fused_form(int dtype) { return g_fused_form[dtype == NLS_F64 ? 1 : 0].load(); }
+
 static int fused_form(int dtype, int order, int bc, long long tiles16, long long tiles32) {
+    const int forced = get_fused_form(dtype);
+    if (forced >= 0) {
+        // forms a configuration does not build run tall (launch_fused_form)
+        if (order == 7 || (forced == FORM_HALF && (dtype != NLS_F64 || bc == BC_REF))) return FORM_TALL;
+        return forced;
+    }
     if (order == 7) return FORM_TALL;
     const int cus = device_cus();
     if (cus <= 0) return FORM_TALL;

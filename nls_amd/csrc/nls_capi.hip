@@ -224,6 +224,7 @@ constexpr size_t GRAPH_CACHE = 8;
 struct GraphEntry {
     int device;
     int stage_ty;          // tile height of the per-stage kernels when captured
+    int fform;             // fused kernel's tile form setting when captured
     nls_problem p;
     const void *P;
     void *psi, *nres, *ws;
@@ -255,9 +256,10 @@ static int block_graph(const nls_problem *p, const void *P, void *psi, void *nre
     GraphCache &gc = *g_graphs;
     // a graph holds the kernels of the launch configuration it was captured
     // with: the per-stage tile height (nls_tune_stage_tile) is part of the key
-    const int ty = get_stage_tile(p->dtype);
+    const int ty = get_stage_tile(p->dtype), ff = get_fused_form(p->dtype);
     for (GraphEntry &e : gc.entries)
-        if (e.device == dev && e.stage_ty == ty && e.P == P && e.psi == psi && e.nres == nres && e.ws == ws &&
+        if (e.device == dev && e.stage_ty == ty && e.fform == ff && e.P == P && e.psi == psi && e.nres == nres &&
+            e.ws == ws &&
             memcmp(&e.p, p, sizeof *p) == 0) {
             e.used = ++gc.clock;
             *out = e.exec;
@@ -296,6 +298,7 @@ static int block_graph(const nls_problem *p, const void *P, void *psi, void *nre
     GraphEntry e;
     e.device = dev;
     e.stage_ty = ty;
+    e.fform = ff;
     e.p = *p;
     e.P = P; e.psi = psi; e.nres = nres; e.ws = ws;
     e.exec = exec;
@@ -600,6 +603,14 @@ int nls_runge_kutta_2d(float dt, float t0, const void *u0, int n, const float *b
     OperatorGiven og;
     og.w = w;
     return operator_dropin(2, n, order, dt, iters, pumping, coeffs, og, u0, u, false);
+}
+
+int nls_tune_fused_form(int dtype, int form) {
+    if (dtype != NLS_F32 && dtype != NLS_F64) return fail(NLS_EINVAL, "bad dtype");
+    if (form == -1) return get_fused_form(dtype) + 1;     // internal -1 (per launch) .. 2 -> 0 .. 3
+    if (form < 0 || form > 3 || set_fused_form(dtype, form - 1))
+        return fail(NLS_EINVAL, "form must be 0 (per launch), 1 (tall), 2 (short) or 3 (half) (got %d)", form);
+    return form;
 }
 
 int nls_tune_stage_tile(int dtype, int rows) {

@@ -17,6 +17,10 @@ int plain_engine(const nls_problem *p);            // RK4 engine of a plain solv
 // the fused kernel holds a member (or row slab) of this many points: its
 // complex array is smaller than 4 GiB (32-bit byte offsets)
 bool fused_holds(long long points, int dtype);
+// process-wide tile form of the fused kernel per dtype: -1 (chosen per
+// launch), 0 tall, 1 short, 2 half (nls_tune_fused_form)
+int set_fused_form(int dtype, int form);
+int get_fused_form(int dtype);
 // plain 2-D RK4 solve of one grid on the fused kernel (nls_batch.hip)
 size_t fused_grid_workspace_bytes(const nls_problem *p);
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,

@@ -38,7 +38,7 @@ EXPORTS = (
     "nls_revervoir_2d", "nls_hamiltonian", "nls_hamiltonian_2d", "nls_runge_kutta", "nls_runge_kutta_2d",
     "nls_batch_set_flags", "nls_batch_observables", "nls_batch_prepare", "nls_batch_energy",
     "nls_batch_update", "nls_batch_observables_async", "nls_batch_energy_async",
-    "nls_clear_first_row_of_derivative", "nls_divide_derivative_on_radius",
+    "nls_clear_first_row_of_derivative", "nls_divide_derivative_on_radius", "nls_tune_fused_form",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2, "hybrid": 3}
 FLAG_EAGER = 1
@@ -175,6 +175,7 @@ def load():
         "nls_batch_observables_async": ([vp, vp, vp, vp, vp], i),
         "nls_batch_energy_async": ([vp, vp, vp, vp, vp, vp], i),
         "nls_tune_stage_tile": ([i, i], i),
+        "nls_tune_fused_form": ([i, i], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -495,6 +496,19 @@ def stage_tile(dtype, rows=0):
     if r < 0:
         _check(r)
     return r
+
+
+FUSED_FORMS = {"auto": 0, "tall": 1, "short": 2, "half": 3}
+
+
+def fused_form(dtype, form=None):
+    """Get (form=None) or set the fused kernel's tile form for dtype:
+    "auto" (per launch), "tall", "short" or "half" (nls_tune_fused_form)."""
+    code = -1 if form is None else FUSED_FORMS[form]
+    r = load().nls_tune_fused_form(DTYPE[dtype] if isinstance(dtype, str) else int(dtype), code)
+    if r < 0:
+        _check(r)
+    return {v: k for k, v in FUSED_FORMS.items()}[r]
 
 
 def workspace_bytes(desc):

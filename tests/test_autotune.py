@@ -70,6 +70,44 @@ def test_stage_tile_and_batch_choices():
     assert sorted({c[1] for c in calls if c[0] == "batch"}) == ["fused", "resident", "stage"]
 
 
+def test_fused_form_candidates_and_choice():
+    T = autotune.Tuner(path=None, device_name="dev",
+                       timer=lambda cand, _s: {"tall": 2.0, "short": 3.0, "half": 1.0}[cand[1]])
+    assert T.form_candidates("f32", "periodic", 5) == ["tall", "short"]
+    assert T.form_candidates("f64", "periodic", 5) == ["tall", "short", "half"]
+    assert T.form_candidates("f64", "ref", 3) == ["tall", "short"]
+    assert T.form_candidates("f64", "periodic", 7) == ["tall"]
+    ms = configs.sweep512(count=4, steps=10)
+    assert T.fused_form(ms, dtype="f32") == "tall"
+    assert T.fused_form(ms, dtype="f64") == "half"
+    d = native.ProblemDesc.make(2, 2048, 5, 1e-4, 0.05, configs.model_coeffs(), bc="periodic", dtype="f64")
+    assert T.plain_fused_form(d) == "half"
+
+
+@pytest.mark.gpu
+def test_tuned_fused_form_batch_matches_each_form(gpu):
+    """The autotuner times every tile form of the fused kernel on a batch,
+    applies the fastest, and a batch created then runs that form; results are
+    the same in every form, bit for bit."""
+    from nls_amd.batch import Batch
+    ms = configs.sweep512(count=24, steps=40)
+    T = autotune.Tuner(path=None, steps=16, reps=1)
+    try:
+        best = T.fused_form(ms, dtype="f32", engine="fused")
+        assert best in ("tall", "short") and set(T.last) == {"tall", "short"} and native.fused_form("f32") == best
+        out = {}
+        for form, ty in (("tall", 32), ("short", 16)):
+            native.fused_form("f32", form)
+            b = Batch(ms, dim=2, order=5, bc="periodic", dtype="f32", engine="fused")
+            assert b.tiles == sum(((m.n + 63) // 64) * ((m.n + ty - 1) // ty) for m in ms)
+            b.advance(40)
+            out[form] = b.psi.cpu().numpy()
+            b.close()
+        np.testing.assert_array_equal(out["tall"], out["short"])
+    finally:
+        native.fused_form("f32", "auto")
+
+
 @pytest.mark.gpu
 def test_tuned_plain_solve_matches_oracle(gpu, oracle, tmp_path):
     T = autotune.Tuner(path=str(tmp_path / "t.json"), steps=64, reps=1)
