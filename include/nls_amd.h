@@ -194,6 +194,19 @@ size_t nls_workspace_bytes(const nls_problem *p);
 int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres,
                 void *workspace, size_t workspace_bytes, void *stream);
 
+/* nls_advance without allocation: the workspace is a device array of
+ * NLS_STATIC_WORKSPACE_BYTES that the library holds on each GPU it runs on
+ * (from its first use there until the process ends), every launch is issued
+ * directly on `stream` (no step-block graph is captured), and no device or
+ * host memory is allocated.  Problems whose nls_workspace_bytes exceeds the
+ * array's size are rejected with NLS_EINVAL.  Calls that overlap on one GPU
+ * share the array and race: run them in stream order.  Same results as
+ * nls_advance.  No counterpart in the reference: for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). */
+#define NLS_STATIC_WORKSPACE_BYTES (64u << 20)
+int nls_advance_static(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *stream);
+
 /* Frees the calling thread's cached step-block graphs (after synchronising
  * the device).  Optional: the cache is bounded. */
 int nls_release(void);

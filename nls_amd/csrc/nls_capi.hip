@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "nls_amd.h"
@@ -341,6 +342,55 @@ int nls_advance(const nls_problem *p, int iters, const void *pumping, void *psi,
     }
     const int rest = iters % GRAPH_BLOCK;
     return rest ? advance_direct(p, rest, pumping, psi, nres, workspace, s) : NLS_OK;
+}
+
+}  // extern "C"
+
+// ---- advance without allocation ------------------------------------------
+// The workspace of nls_advance_static: a file-scope device array, one copy on
+// every GPU the library runs on, held until the process ends.  Host code never
+// passes the array's name as an address: its address on the current device
+// comes from hipGetSymbolAddress, taken once per device (static_workspace).
+// Every call rewrites what it later reads of the array (state ping-pong,
+// stage buffers, member descriptor, radial and FFT tables) before reading it,
+// so nothing needs clearing; calls that overlap on one GPU share the array
+// and race.
+static __device__ __align__(256) unsigned char scratch[NLS_STATIC_WORKSPACE_BYTES];
+
+static int static_workspace(void **out) {
+    static std::mutex mu;
+    static void *addr[64];                 // per device
+    int dev = 0;
+    int rc = hip_check(hipGetDevice(&dev), "current device");
+    if (rc) return rc;
+    if (dev < 0 || dev >= 64) return fail(NLS_EUNSUPPORTED, "device %d: at most 64 devices", dev);
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void *a = nullptr;
+        if ((rc = hip_check(hipGetSymbolAddress(&a, HIP_SYMBOL(scratch)), "static workspace address"))) return rc;
+        addr[dev] = a;
+    }
+    *out = addr[dev];
+    return NLS_OK;
+}
+
+extern "C" {
+
+int nls_advance_static(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *stream) {
+    int rc = check_problem(p);
+    if (rc) return rc;
+    if (iters < 0) return fail(NLS_EINVAL, "iters must be >= 0");
+    const bool dyn = p->coeffs[10] != 0.0;
+    if (!pumping || !psi || (dyn && !nres)) return fail(NLS_EINVAL, "NULL device buffer");
+    const size_t need = nls_workspace_bytes(p);
+    if (need > (size_t)NLS_STATIC_WORKSPACE_BYTES)
+        return fail(NLS_EINVAL, "this problem needs a workspace of %zu bytes; nls_advance_static holds at most %zu "
+                    "(nls_advance takes a workspace of the caller's)", need, (size_t)NLS_STATIC_WORKSPACE_BYTES);
+    if (iters == 0) return NLS_OK;
+    void *ws = nullptr;
+    if ((rc = static_workspace(&ws))) return rc;
+    // direct launches only: no step-block graph is captured or cached
+    return advance_direct(p, iters, pumping, psi, nres, ws, (hipStream_t)stream);
 }
 
 int nls_slab_halo_rows(int order) {

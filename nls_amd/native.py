@@ -37,7 +37,7 @@ EXPORTS = (
     "nls_make_banded_matrix", "nls_make_laplacian_2d", "nls_rgbmv", "nls_rbbmv", "nls_revervoir",
     "nls_revervoir_2d", "nls_hamiltonian", "nls_hamiltonian_2d", "nls_runge_kutta", "nls_runge_kutta_2d",
     "nls_batch_set_flags", "nls_batch_observables", "nls_batch_prepare", "nls_batch_energy",
-    "nls_batch_update", "nls_batch_observables_async", "nls_batch_energy_async",
+    "nls_batch_update", "nls_batch_observables_async", "nls_batch_energy_async", "nls_advance_static",
     "nls_clear_first_row_of_derivative", "nls_divide_derivative_on_radius", "nls_tune_fused_form",
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2, "hybrid": 3}
@@ -139,6 +139,7 @@ def load():
         "nls_solve_host": ([pp, i, vp, vp, vp], i),
         "nls_workspace_bytes": ([pp], sz),
         "nls_advance": ([pp, i, vp, vp, vp, vp, sz, vp], i),
+        "nls_advance_static": ([pp, i, vp, vp, vp, vp], i),
         "nls_release": ([], i),
         "nls_slab_halo_rows": ([i], i),
         "nls_make_banded_matrix": ([i, i, vp, vp], i),
@@ -528,6 +529,18 @@ def advance(desc, iters, pumping, psi, nres, workspace, stream=None):
     _check(load().nls_advance(C.byref(desc), int(iters), _dp(pumping), _dp(psi), _dp(nres),
                               _dp(workspace), workspace.numel() * workspace.element_size() if workspace is not None else 0,
                               C.c_void_p(s.cuda_stream)))
+
+
+STATIC_WORKSPACE_BYTES = 64 << 20      # NLS_STATIC_WORKSPACE_BYTES
+
+
+def advance_static(desc, iters, pumping, psi, nres, stream=None):
+    """Device solve without allocation (nls_advance_static): the library's
+    own per-device workspace array, direct launches only."""
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    _check(load().nls_advance_static(C.byref(desc), int(iters), _dp(pumping), _dp(psi), _dp(nres),
+                                     C.c_void_p(s.cuda_stream)))
 
 
 def release():
