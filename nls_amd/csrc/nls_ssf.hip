@@ -18,11 +18,20 @@
 // 1-D: a workgroup holds a whole line (n <= 4096) in LDS and runs all steps
 // on chip.
 //
-// FFTs: in-place Stockham radix-4 (last stage radix-2 when log2 n is odd) on
-// the complex values a workgroup holds in LDS (2048 in the 2-D passes, 4096
-// for a 1-D line); each stage stages its butterfly inputs and twiddles in
-// registers, synchronises, then writes the outputs in place.  Twiddles exp(-2 pi i m / n) and the phase table are generated in
-// double precision and rounded once to the working precision.
+// Precision: the fields sit in HBM in the problem's dtype; every kernel
+// computes in double precision (FFT stages, pointwise RK4, twiddle and phase
+// tables), with a single-precision state converted on load and rounded once
+// on store.  On MI355X double-precision vector arithmetic issues at the
+// single-precision rate (packed FP32 aside), so single precision only saves
+// HBM bytes; rounding the state once per pass instead of at every FFT stage
+// keeps its error far below that of single-precision transforms.
+//
+// FFTs: in-place Stockham stages, radix-8 while at least three factors of two
+// remain, then radix-4 and a last radix-2 as needed, on the complex values a
+// workgroup holds in LDS (2048 in the 2-D passes, 4096 for a 1-D line); each
+// stage stages its butterfly inputs and twiddles in registers, synchronises,
+// then writes the outputs in place.  Twiddles exp(-2 pi i m / n) and the
+// phase table are generated in double precision.
 #include <cstring>
 
 #include "nls_core.h"
@@ -46,6 +55,27 @@ template <typename T> struct ColShape;
 template <> struct ColShape<double> { static constexpr int LOG_L = 1, PTS = 4096, NTH = 512; };
 template <> struct ColShape<float> { static constexpr int LOG_L = 2, PTS = 8192, NTH = 1024; };
 
+// LDS layout of the transforms: element i at i + (i >> SHIFT), one complex of
+// padding per 2^SHIFT, so that the strided writes of the first radix-8
+// stages (stride 8, then groups of 8 at stride 64) spread over the LDS banks
+// instead of piling on a few; SHIFT = 31 is the plain layout.  Each kernel
+// takes its own shift (the padding costs index arithmetic where the
+// conflicts it removes are few).
+#ifndef NLS_SSF_ROW_PAD
+#define NLS_SSF_ROW_PAD 4
+#endif
+#ifndef NLS_SSF_COL_PAD
+#define NLS_SSF_COL_PAD 4
+#endif
+#ifndef NLS_SSF_LINE_PAD
+#define NLS_SSF_LINE_PAD 4
+#endif
+constexpr int lds_padded(int pts, int shift) { return pts + (shift < 31 ? pts >> shift : 0); }
+template <typename T, int SHIFT> struct Lds {
+    cplx_t<T> *p;
+    __device__ __forceinline__ cplx_t<T> &operator[](int i) const { return p[SHIFT < 31 ? i + (i >> SHIFT) : i]; }
+};
+
 template <typename T>
 __device__ __forceinline__ cplx_t<T> cmul(cplx_t<T> a, cplx_t<T> b) {
     return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
@@ -57,8 +87,8 @@ __device__ __forceinline__ cplx_t<T> cmul(cplx_t<T> a, cplx_t<T> b) {
 // butterfly inputs from LDS and its twiddles from the table (global memory,
 // cached) before the barrier, so both are in flight together; after the
 // barrier it computes and writes the outputs in place.
-template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
-__device__ __forceinline__ void fft_radix4_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH, typename LB>
+__device__ __forceinline__ void fft_radix4_stage(LB buf, int logN, int logL, int lstride, int estride,
                                                  int logNs, const cplx_t<T> *tw) {
     using C = cplx_t<T>;
     constexpr int QB = (PTS / 4 + NTH - 1) / NTH;    // butterflies per thread
@@ -128,8 +158,8 @@ __device__ __forceinline__ void dft4(cplx_t<T> &a0, cplx_t<T> &a1, cplx_t<T> &a2
 }
 
 // w8 = exp(-2 pi i / 8), the twiddle table's entry N/8.
-template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
-__device__ __forceinline__ void fft_radix8_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH, typename LB>
+__device__ __forceinline__ void fft_radix8_stage(LB buf, int logN, int logL, int lstride, int estride,
                                                  int logNs, const cplx_t<T> *tw, cplx_t<T> w8) {
     using C = cplx_t<T>;
     constexpr int QB = (PTS / 8 + NTH - 1) / NTH;
@@ -190,8 +220,8 @@ __device__ __forceinline__ void fft_radix8_stage(cplx_t<T> *buf, int logN, int l
     __syncthreads();
 }
 
-template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
-__device__ __forceinline__ void fft_radix2_stage(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH, typename LB>
+__device__ __forceinline__ void fft_radix2_stage(LB buf, int logN, int logL, int lstride, int estride,
                                                  int logNs, const cplx_t<T> *tw) {
     using C = cplx_t<T>;
     constexpr int QB = (PTS / 2 + NTH - 1) / NTH;
@@ -232,17 +262,12 @@ __device__ __forceinline__ void fft_radix2_stage(cplx_t<T> *buf, int logN, int l
 }
 
 // In-place FFT of 2^logL lines of length 2^logN (2^(logL+logN) <= PTS) in LDS.
-template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH>
-__device__ __forceinline__ void lds_fft(cplx_t<T> *buf, int logN, int logL, int lstride, int estride,
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH, typename LB>
+__device__ __forceinline__ void lds_fft(LB buf, int logN, int logL, int lstride, int estride,
                                         const cplx_t<T> *tw) {
     int logNs = 0;
     if (sizeof(T) == 8 && logN >= 3) {
-        // double precision: radix-8 stages (2048 = 8 8 8 4: four passes
-        // through LDS instead of six).  In single precision the radix-4
-        // stages are kept: over 2e4 steps of a 2048^2 grid their round-off
-        // accumulates to 1.2e-3 of the FP64 solution against 1.8e-3 with
-        // radix-8 stages (the round-off of each transform repeats from step
-        // to step, as the data changes slowly, and grows linearly).
+        // radix-8 stages (2048 = 8 8 8 4: four passes through LDS instead of six)
         const cplx_t<T> w8 = tw[(1 << logN) >> 3];
         do {
             fft_radix8_stage<T, INV, LINE_FAST, PTS, NTH>(buf, logN, logL, lstride, estride, logNs, tw, w8);
@@ -279,17 +304,30 @@ __device__ __forceinline__ void pointwise_rk4(const Coeffs<T> &c, T inv_c11, T P
     if (DYN) n = n + (m1 + (T)2 * m2 + (T)2 * m3 + m4) * t6;
 }
 
-template <typename T> struct SsfArgs {
+// S: the precision of the fields in HBM (the problem's dtype); T: the
+// arithmetic, LDS and tables, double in both cases -- on this GPU double
+// precision vector arithmetic runs at the single-precision rate (no packed
+// FP32), so a single-precision state only halves the HBM bytes and is
+// rounded once per pass, not at every FFT stage.
+template <typename S, typename T> struct SsfArgs {
     int logN;
     T half_dt, inv_c11;
     Coeffs<T> c;
-    cplx_t<T> *psi;
-    T *nres;
-    const T *P;
+    cplx_t<S> *psi;
+    S *nres;
+    const S *P;
     const cplx_t<T> *tw;      // exp(-2 pi i m / N)
     const cplx_t<T> *ph;      // exp(-i k_m^2 dt) / N
     int iters;                // 1-D kernel only
 };
+
+// A stored complex value in the arithmetic precision, and back.
+template <typename T, typename S> __device__ __forceinline__ cplx_t<T> widen(cplx_t<S> v) {
+    return mk<T>((T)v.x, (T)v.y);
+}
+template <typename S, typename T> __device__ __forceinline__ cplx_t<S> narrow(cplx_t<T> v) {
+    return mk<S>((S)v.x, (S)v.y);
+}
 
 enum { ROWS_FIRST = 0, ROWS_MIDDLE = 1, ROWS_LAST = 2 };
 
@@ -299,11 +337,12 @@ __host__ __device__ constexpr int lines_per_wg(int logN, int logP) {
 }
 
 // Row pass: 2^(11-logN) rows per workgroup (at most n).
-template <typename T, bool DYN, int MODE>
-__global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<T> a) {
+template <typename S, typename T, bool DYN, int MODE>
+__global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<S, T> a) {
     using C = cplx_t<T>;
     constexpr int PPT = ROW_PTS / ROW_NTH;
-    __shared__ C buf[ROW_PTS];
+    __shared__ C buf_mem[lds_padded(ROW_PTS, NLS_SSF_ROW_PAD)];
+    const Lds<T, NLS_SSF_ROW_PAD> buf{buf_mem};
     const int logN = a.logN, N = 1 << logN;
     const int logL = lines_per_wg(logN, ROW_LOG_PTS);
     const int pts = 1 << (logL + logN);
@@ -314,9 +353,9 @@ __global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<T> a) {
         const int idx = threadIdx.x + ROW_NTH * q;
         pp[q] = 0; nn[q] = 0;
         if (idx < pts) {
-            buf[idx] = a.psi[base + idx];
-            pp[q] = a.P[base + idx];
-            if (DYN) nn[q] = a.nres[base + idx];
+            buf[idx] = widen<T, S>(a.psi[base + idx]);
+            pp[q] = (T)a.P[base + idx];
+            if (DYN) nn[q] = (T)a.nres[base + idx];
         }
     }
     __syncthreads();
@@ -348,8 +387,8 @@ __global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<T> a) {
     for (int q = 0; q < PPT; ++q) {
         const int idx = threadIdx.x + ROW_NTH * q;
         if (idx < pts) {
-            a.psi[base + idx] = buf[idx];
-            if (DYN) a.nres[base + idx] = nn[q];
+            a.psi[base + idx] = narrow<S, T>(buf[idx]);
+            if (DYN) a.nres[base + idx] = (S)nn[q];
         }
     }
 }
@@ -357,13 +396,14 @@ __global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<T> a) {
 // Column pass: min(2^LOG_L, n) adjacent columns per workgroup, LDS
 // [row][col].  Workgroups of one XCD take adjacent column blocks
 // (round-robin dispatch).
-template <typename T>
-__global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<T> a, int nblocks, int chunk) {
+template <typename S, typename T>
+__global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<S, T> a, int nblocks, int chunk) {
     using C = cplx_t<T>;
     constexpr int COL_PTS = ColShape<T>::PTS, COL_NTH = ColShape<T>::NTH;
     constexpr int PPT = COL_PTS / COL_NTH;
     static_assert(COL_PTS == 2048 << ColShape<T>::LOG_L, "column block of 2048-point columns");
-    __shared__ C buf[COL_PTS];
+    __shared__ C buf_mem[lds_padded(COL_PTS, NLS_SSF_COL_PAD)];
+    const Lds<T, NLS_SSF_COL_PAD> buf{buf_mem};
     const int b = blockIdx.x;
     const int blk = (b & 7) * chunk + (b >> 3);
     if (blk >= nblocks) return;
@@ -376,7 +416,7 @@ __global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<T> 
         const int idx = threadIdx.x + COL_NTH * q;
         if (idx < pts) {
             const int m = idx >> logL, cc = idx & (L - 1);
-            buf[idx] = a.psi[(size_t)m * N + col0 + cc];
+            buf[idx] = widen<T, S>(a.psi[(size_t)m * N + col0 + cc]);
         }
     }
     __syncthreads();
@@ -400,19 +440,20 @@ __global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<T> 
         const int idx = threadIdx.x + COL_NTH * q;
         if (idx < pts) {
             const int m = idx >> logL, cc = idx & (L - 1);
-            a.psi[(size_t)m * N + col0 + cc] = buf[idx];
+            a.psi[(size_t)m * N + col0 + cc] = narrow<S, T>(buf[idx]);
         }
     }
 }
 
 // 1-D: one workgroup holds the whole line in LDS and runs every step; the
 // pumping and reservoir of the thread's points stay in registers.
-template <typename T, bool DYN>
-__global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<T> a) {
+template <typename S, typename T, bool DYN>
+__global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<S, T> a) {
     using C = cplx_t<T>;
     constexpr int LINE_NTH = LineShape<T>::NTH;
     constexpr int PPT = LINE_PTS / LINE_NTH;
-    __shared__ C buf[LINE_PTS];
+    __shared__ C buf_mem[lds_padded(LINE_PTS, NLS_SSF_LINE_PAD)];
+    const Lds<T, NLS_SSF_LINE_PAD> buf{buf_mem};
     const int logN = a.logN, N = 1 << logN;
     T pp[PPT], nn[PPT];
     C ph[PPT];                                       // the phase of each point, for every step
@@ -422,14 +463,14 @@ __global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<T>
         pp[q] = 0; nn[q] = 0; ph[q] = mk<T>(0, 0);
         if (idx < N) {
             ph[q] = a.ph[idx];
-            pp[q] = a.P[idx];
-            if (DYN) nn[q] = a.nres[idx];
+            pp[q] = (T)a.P[idx];
+            if (DYN) nn[q] = (T)a.nres[idx];
         }
     }
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int idx = threadIdx.x + LINE_NTH * q;
-        if (idx < N) buf[idx] = a.psi[idx];
+        if (idx < N) buf[idx] = widen<T, S>(a.psi[idx]);
     }
     for (int it = 0; it < a.iters; ++it) {
         // each thread only touches its own points here, so no barrier is
@@ -467,8 +508,8 @@ __global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<T>
     for (int q = 0; q < PPT; ++q) {
         const int idx = threadIdx.x + LINE_NTH * q;
         if (idx < N) {
-            a.psi[idx] = buf[idx];
-            if (DYN) a.nres[idx] = nn[q];
+            a.psi[idx] = narrow<S, T>(buf[idx]);
+            if (DYN) a.nres[idx] = (S)nn[q];
         }
     }
 }
@@ -520,15 +561,16 @@ static int log2_exact(int n) {
     return (1 << l) == n ? l : -1;
 }
 
-template <typename T, bool DYN>
-static int ssf_run(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, T *nres, char *ws, hipStream_t s) {
+template <typename S, bool DYN>
+static int ssf_run(const nls_problem *p, int iters, const S *P, cplx_t<S> *psi, S *nres, char *ws, hipStream_t s) {
+    using T = double;                 // arithmetic, LDS and tables
     using C = cplx_t<T>;
     const int N = p->n, logN = log2_exact(N);
     C *tw = (C *)ws, *ph = (C *)(ws + align256s(N * sizeof(C)));
     // 1/N per inverse transform: once in 1-D, once per axis (folded into the
     // product ph[ky] * ph[kx]) in 2-D
     hipLaunchKernelGGL(k_ssf_tables<T>, dim3((N + 255) / 256), dim3(256), 0, s, N, p->dx, p->dt, 1.0 / N, tw, ph);
-    SsfArgs<T> a;
+    SsfArgs<S, T> a;
     memset(&a, 0, sizeof a);
     a.logN = logN;
     a.half_dt = (T)(0.5 * p->dt);
@@ -537,26 +579,25 @@ static int ssf_run(const nls_problem *p, int iters, const T *P, cplx_t<T> *psi, 
     a.psi = psi; a.nres = nres; a.P = P; a.tw = tw; a.ph = ph;
     if (p->dim == 1) {
         a.iters = iters;
-        hipLaunchKernelGGL((k_ssf_line<T, DYN>), dim3(1), dim3(LineShape<T>::NTH), 0, s, a);
+        hipLaunchKernelGGL((k_ssf_line<S, T, DYN>), dim3(1), dim3(LineShape<T>::NTH), 0, s, a);
         return hip_check(hipGetLastError(), "ssf line launch");
     }
     const int rblocks = N >> lines_per_wg(logN, ROW_LOG_PTS);
     const int cblocks = N >> (ColShape<T>::LOG_L < logN ? ColShape<T>::LOG_L : logN);
     const int chunk = (cblocks + 7) / 8;
-    hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_FIRST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
+    hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_FIRST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL(k_ssf_cols<T>, dim3(8 * chunk), dim3(ColShape<T>::NTH), 0, s, a, cblocks, chunk);
+        hipLaunchKernelGGL((k_ssf_cols<S, T>), dim3(8 * chunk), dim3(ColShape<T>::NTH), 0, s, a, cblocks, chunk);
         if (it + 1 < iters)
-            hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_MIDDLE>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
+            hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_MIDDLE>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
         else
-            hipLaunchKernelGGL((k_ssf_rows<T, DYN, ROWS_LAST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
+            hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_LAST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
     }
     return hip_check(hipGetLastError(), "ssf launch");
 }
 
 size_t ssf_workspace_bytes(const nls_problem *p) {
-    const size_t cs = p->dtype == NLS_F64 ? 16 : 8;
-    return 2 * align256s((size_t)p->n * cs);
+    return 2 * align256s((size_t)p->n * 16);          // twiddles and phases, double precision
 }
 
 int ssf_check(const nls_problem *p) {
