@@ -390,9 +390,12 @@ def serving(rates, capacity_ref=None):
                                  s.get("offered_grid_point_steps_per_s", 0) / 1e9))
         runs.append(s)
     mean_work = S.mean_request_work()
-    cap = max(r.get("steady_grid_point_steps_per_s", 0.0) for r in runs)
-    out = {"runs": runs, "mean_request_grid_point_steps": mean_work, "capacity_grid_point_steps_per_s": cap,
-           "saturation_rate_req_s": cap / mean_work}
+    sat = S.capacity()
+    cap = sat["grid_point_steps_per_s"]
+    log("serving saturated: %d requests at once, %.1f G grid-point-steps/s until half the work is done (%.1f G "
+        "with the tail)" % (sat["requests"], cap / 1e9, sat["grid_point_steps_per_s_incl_tail"] / 1e9))
+    out = {"runs": runs, "saturated": sat, "mean_request_grid_point_steps": mean_work,
+           "capacity_grid_point_steps_per_s": cap, "saturation_rate_req_s": cap / mean_work}
     if capacity_ref:
         out["capacity_vs_batch_engine"] = cap / capacity_ref
     log("serving capacity %.1f G grid-point-steps/s = %.0f requests/s of the mix%s" % (

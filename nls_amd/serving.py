@@ -537,6 +537,33 @@ def serve(rate, window=None, budget_gb=16, seed=0, step_range=(1e3, 1e5), device
     return s
 
 
+def capacity(count=400, budget_gb=16, seed=7, step_range=(1e3, 1e5), device_inputs=True):
+    """Saturated throughput of the request loop: `count` requests of the
+    serving mix all arriving at once, so every pool runs as full as the HBM
+    budget lets it.  The rate is the work of the blocks collected until half
+    of all the work is done, per second: the time when the loop is loaded,
+    before the tail in which only the longest requests remain."""
+    import torch
+    rng = np.random.default_rng(seed)
+    reqs = [make_request(i, 0.0, rng, step_range, device_inputs=device_inputs) for i in range(count)]
+    total = sum(r.points * r.steps for r in reqs)
+    eng = Engine(budget_bytes=int(budget_gb * (1 << 30)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = eng.run(reqs)
+    wall = time.perf_counter() - t0
+    eng.close()
+    acc, t_half = 0, None
+    for t, w in eng.block_log:
+        acc += w
+        if acc >= total / 2:
+            t_half = t
+            break
+    return {"requests": len(done), "failed": sum(r.error is not None for r in done), "work": total,
+            "wall_s": wall, "half_work_s": t_half, "grid_point_steps_per_s": (total / 2) / t_half,
+            "grid_point_steps_per_s_incl_tail": total / wall}
+
+
 def sweep(rates=(20, 50, 100, 200, 500), budget_gb=16, seed=0, step_range=(1e3, 1e5), log=print):
     out = []
     for lam in rates:
