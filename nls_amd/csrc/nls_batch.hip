@@ -528,6 +528,251 @@ k_fused_step_2d(const FusedArgs<T> f) {
     }
 }
 
+// Single precision, order 5, periodic grids: the same step with each thread
+// owning vertical pairs of points, (x, y) and (x, y + 1).  Every ring of the region is two
+// rows tall and every stage's rectangle starts on an even row, so pairs tile
+// all of them; listed in ring order (PairMap) each stage's rectangle is still
+// a prefix.  The two points of a pair share the column of six values their
+// stencils read (rows y - 2 .. y + 3): 14 LDS reads a pair instead of 18, and
+// one address.  A point's arithmetic is the same expression as in
+// k_fused_step_2d, so results are the same bit for bit.
+#ifndef NLS_FUSED_PAIRS
+#define NLS_FUSED_PAIRS 1
+#endif
+
+template <int W, int HH, int WS> struct PairMap {
+    unsigned v[W * HH / 2];
+    constexpr PairMap() : v() {
+        int k = 0;
+        constexpr int R = 2;
+        for (int s = 4; s >= 0; --s) {
+            const int a = s * R, b = (s + 1) * R;
+            for (int y = a; y < HH - a; y += 2)
+                for (int x = a; x < W - a; ++x) {
+                    const bool inner = s < 4 && y >= b && y < HH - b && x >= b && x < W - b;
+                    if (!inner) v[k++] = (unsigned)(y * WS + x) | (unsigned)x << 16 | (unsigned)y << 24;
+                }
+        }
+    }
+};
+static_assert(PairMap<80, 48, 84>().v[0] == (8 * 84 + 8 | 8 << 16 | 8 << 24), "pair order starts at the output tile");
+static_assert(PairMap<80, 48, 84>().v[1024] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the tile");
+static_assert(PairMap<80, 32, 84>().v[512] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the short tile");
+template <int W, int HH, int WS> __device__ const PairMap<W, HH, WS> pair_map{};
+
+template <int BC, bool DYN, int FORM>
+__global__ void __launch_bounds__(1024, 8) k_fused_pairs_2d(const FusedArgs<float> f) {
+    using T = float;
+    using C = float2;
+    using FT = FusedTile<float, 5, FORM>;
+    constexpr int R = 2, H = 4 * R;
+    constexpr int TX = 64, TY = FT::TY, NT = FT::NT;
+    constexpr int W = TX + 2 * H, HH = TY + 2 * H, NPP = W * HH / 2;
+    constexpr int WS = fused_lds_stride(W, R);
+    constexpr int J = (NPP + NT - 1) / NT;          // pairs per thread
+    constexpr int NOUT = TX * TY / 2;              // output pairs: the first NOUT entries
+    static_assert(TY % 2 == 0 && HH % 2 == 0 && W < 256 && HH < 256, "pair tile shape");
+    constexpr int PAD = R * WS + R;
+    __shared__ C ys_mem[2][HH * WS + 2 * PAD];
+    C *const ya = ys_mem[0] + PAD;
+    C *const yb = ys_mem[1] + PAD;
+
+    const int t = xcd_tile(f.ntiles, f.chunk);
+    if (t < 0) return;
+    TileDesc td;
+    MemberDev<T> m;
+    if (f.grid_ntx > 0) {
+        td.m = 0; td.x0 = (t % f.grid_ntx) * TX; td.y0 = (t / f.grid_ntx) * TY + (BC == BC_SLAB ? H : 0);
+        m = f.grid;
+    } else {
+        td = f.tiles[t];
+        m = f.mem[td.m];
+    }
+    const int n = m.n;
+    const int ylim = BC == BC_SLAB ? f.slab_end : n;
+    const long long off = m.off;
+    const C *u = f.u + off;
+    const T *un = DYN ? f.un + off : nullptr;
+    const T *P = f.P + off;
+    C *uo = f.uo + off;
+    T *uno = DYN ? f.uno + off : nullptr;
+    const int tid = threadIdx.x;
+    const unsigned cbytes = (unsigned)((unsigned long long)(BC == BC_SLAB ? f.slab_end + H : n) * n * sizeof(C));
+    const __amdgpu_buffer_rsrc_t ru = buffer_of(u, cbytes), rp = buffer_of(P, cbytes / 2),
+                                 rn = buffer_of(DYN ? un : P, cbytes / 2);
+
+    if (f.it >= m.steps) {            // budget used up: pass the state through
+        for (int k = tid; k < TX * TY; k += NT) {
+            const int gx = td.x0 + (k & (TX - 1)), gy = td.y0 + k / TX;
+            if (gx >= n || gy >= ylim) continue;
+            const size_t gi = (size_t)gy * n + gx;
+            uo[gi] = u[gi];
+            if (DYN) uno[gi] = un[gi];
+        }
+        return;
+    }
+
+    const T dt = m.dt;
+    const T dt2 = dt * (T)0.5;
+    const T dt6 = dt / (T)6;
+    Coeffs<T> c = m.c;
+    asm volatile("" : "+v"(c.c4), "+v"(c.d13), "+v"(c.c13));
+    T wc = (T)2 * m.w0;                // centre weight of the 2-D Laplacian, joins the point's factor
+    asm volatile("" : "+v"(wc));
+    const T w1 = m.w1, w2 = m.w2;
+    const unsigned *map = pair_map<W, HH, WS>.v;
+
+    C uu[J][2], acc[J][2];
+    T cp[J][2], nn[J][2], yn[J][2], accn[J][2];
+    int li[J];                         // LDS index of the pair's upper point
+    unsigned long long inside = 0;     // bit 2 j + q: point q of pair j lies on the grid
+    static_assert(2 * J <= 64, "inside mask");
+    const bool near = n >= TX + H;
+
+    unsigned ent[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int k = tid + NT * j;
+        ent[j] = (k >= NOUT && (NPP % NT == 0 || j < J - 1 || k < NPP)) ? map[k] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        const int k = tid + NT * j;
+        li[j] = 0;
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            uu[j][q] = mk<T>(0, 0); cp[j][q] = 0; nn[j][q] = 0; acc[j][q] = mk<T>(0, 0); accn[j][q] = 0;
+        }
+        if (NPP % NT == 0 || j < J - 1 || k < NPP) {
+            int lx, ly;
+            if (k < NOUT) {
+                lx = H + (k & (TX - 1));
+                ly = H + 2 * (k / TX);
+                li[j] = ly * WS + lx;
+            } else {
+                const unsigned e = ent[j];
+                li[j] = e & 0xffff;
+                lx = (int)((e >> 16) & 255);
+                ly = (int)(e >> 24);
+            }
+            int gx = td.x0 - H + lx;
+            if (BC == BC_PERIODIC || BC == BC_SLAB) gx = near ? wrap_near3(gx, n) : wrap_index(gx, n);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                int gy = td.y0 - H + ly + q;
+                bool ok = true;
+                if (BC == BC_PERIODIC) gy = near ? wrap_near3(gy, n) : wrap_index(gy, n);
+                else if (BC == BC_SLAB) ok = gy < ylim + H;
+                else ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
+                if (ok) {
+                    const unsigned gi = (unsigned)(gy * n + gx);
+                    inside |= 1ull << (2 * j + q);
+                    uu[j][q] = buffer_load_c<T>(ru, gi * (unsigned)sizeof(C));
+                    cp[j][q] = buffer_load_r<T>(rp, gi * (unsigned)sizeof(T));
+                    if (DYN) nn[j][q] = buffer_load_r<T>(rn, gi * (unsigned)sizeof(T));
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+        if (NPP % NT == 0 || j < J - 1 || tid + NT * j < NPP) {
+            ya[li[j]] = uu[j][0];
+            ya[li[j] + WS] = uu[j][1];
+        }
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            cp[j][q] = pump_term<T, DYN>(c, cp[j][q]);
+            yn[j][q] = nn[j][q];
+        }
+    }
+    __syncthreads();
+
+    typedef T V2 __attribute__((ext_vector_type(2)));
+    using LV = __attribute__((address_space(3))) const V2;
+    // right-hand sides of the pair's two points from stage input yin
+    auto pair_rhs = [&](const C *yin, int j, C k[2], T kn[2]) {
+        // LDS byte address of the column's top (R rows above the upper point),
+        // kept opaque: every read is an immediate offset from it
+        unsigned a = (unsigned)(size_t)((LV *)yin + (li[j] - R * WS));
+        asm volatile("" : "+v"(a));
+        LV *ys = (LV *)(size_t)a;
+        auto rd = [&](int i) { const V2 v = ys[i]; return mk<T>(v.x, v.y); };
+        const C c0 = rd(0), c1 = rd(WS), yA = rd(2 * WS), yB = rd(3 * WS), c4 = rd(4 * WS), c5 = rd(5 * WS);
+        const C a1m = rd(2 * WS - 1), a1p = rd(2 * WS + 1), a2m = rd(2 * WS - 2), a2p = rd(2 * WS + 2);
+        // the stencil sum as in k_fused_step_2d: (horizontal pair + vertical pair) per distance
+        T lr = w1 * ((a1m.x + a1p.x) + (c1.x + yB.x));
+        T lm = w1 * ((a1m.y + a1p.y) + (c1.y + yB.y));
+        lr += w2 * ((a2m.x + a2p.x) + (c0.x + c4.x));
+        lm += w2 * ((a2m.y + a2p.y) + (c0.y + c4.y));
+        kn[0] = 0;
+        rhs_point_centre<T, DYN>(c, cp[j][0], wc, yA, yn[j][0], mk<T>(lr, lm), k[0], kn[0]);
+        // the lower point's row after the upper point is done, so that its
+        // reads do not all live in registers at once
+        __builtin_amdgcn_sched_barrier(0);
+        const C b1m = rd(3 * WS - 1), b1p = rd(3 * WS + 1), b2m = rd(3 * WS - 2), b2p = rd(3 * WS + 2);
+        lr = w1 * ((b1m.x + b1p.x) + (yA.x + c4.x));
+        lm = w1 * ((b1m.y + b1p.y) + (yA.y + c4.y));
+        lr += w2 * ((b2m.x + b2p.x) + (c1.x + c5.x));
+        lm += w2 * ((b2m.y + b2p.y) + (c1.y + c5.y));
+        kn[1] = 0;
+        rhs_point_centre<T, DYN>(c, cp[j][1], wc, yB, yn[j][1], mk<T>(lr, lm), k[1], kn[1]);
+    };
+
+#pragma unroll
+    for (int s = 1; s <= 3; ++s) {
+        const int cnt = (W - 2 * s * R) * (HH - 2 * s * R) / 2;
+        int lim = cnt;
+        asm volatile("" : "+s"(lim));
+        const T dta = (s == 3) ? dt : dt2;
+        const C *yin = (s == 2) ? yb : ya;
+        C *yout = (s == 2) ? ya : yb;
+#pragma unroll
+        for (int j = 0; j < J; ++j) {
+            if (NT * j >= cnt) break;
+            if (tid + NT * j >= lim) continue;
+            C k[2];
+            T kn[2];
+            pair_rhs(yin, j, k, kn);
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                if (s == 1) {
+                    acc[j][q] = k[q];
+                    if (DYN) accn[j][q] = kn[q];
+                } else {
+                    acc[j][q].x = acc[j][q].x + (T)2 * k[q].x;
+                    acc[j][q].y = acc[j][q].y + (T)2 * k[q].y;
+                    if (DYN) accn[j][q] = accn[j][q] + (T)2 * kn[q];
+                }
+                C yv = mk<T>(uu[j][q].x + k[q].x * dta, uu[j][q].y + k[q].y * dta);
+                if (BC != BC_PERIODIC && !((inside >> (2 * j + q)) & 1ull)) yv = mk<T>(0, 0);
+                yout[li[j] + q * WS] = yv;
+                if (DYN) yn[j][q] = nn[j][q] + kn[q] * dta;
+            }
+        }
+        __syncthreads();
+    }
+
+    // stage 4 on the output tile and u_new = u + (k1 + 2 k2 + 2 k3 + k4) dt/6
+#pragma unroll
+    for (int j = 0; j < (NOUT + NT - 1) / NT; ++j) {
+        const int k0 = tid + NT * j;
+        if (k0 >= NOUT) break;
+        const int gx = td.x0 + (k0 & (TX - 1)), gy = td.y0 + 2 * (k0 / TX);
+        if (gx >= n || gy >= ylim) continue;
+        C k[2];
+        T kn[2];
+        pair_rhs(yb, j, k, kn);
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+            if (gy + q >= ylim) break;
+            const size_t gi = (size_t)(gy + q) * n + gx;
+            uo[gi] = mk<T>(uu[j][q].x + (acc[j][q].x + k[q].x) * dt6, uu[j][q].y + (acc[j][q].y + k[q].y) * dt6);
+            if (DYN) uno[gi] = nn[j][q] + (accn[j][q] + kn[q]) * dt6;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Engine RESIDENT: one workgroup per member holds the member's
 // whole grid on chip (1-D n <= 4096, 2-D n <= 64) and runs all of the call's
@@ -753,6 +998,15 @@ static void launch_batch_stage(int dim, int order, int bc, bool dyn, const Batch
 template <typename T, int O, int B, bool D>
 static void launch_fused_form(int form, const FusedArgs<T> &f, hipStream_t s) {
     const dim3 grid(8 * f.chunk);
+    // periodic grids (the variants with reflecting boundaries or row slabs
+    // would spill registers in pairs and keep one point per entry)
+    if constexpr (sizeof(T) == 4 && O == 5 && B == BC_PERIODIC && NLS_FUSED_PAIRS) {
+        if (form == FORM_SHORT)
+            hipLaunchKernelGGL((k_fused_pairs_2d<B, D, FORM_SHORT>), grid, dim3(1024), 0, s, f);
+        else
+            hipLaunchKernelGGL((k_fused_pairs_2d<B, D, FORM_TALL>), grid, dim3(1024), 0, s, f);
+        return;
+    }
     if constexpr (O != 7) {
         if (form == FORM_SHORT) {
             hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_SHORT>), grid, dim3(FusedTile<T, O, FORM_SHORT>::NT),
