@@ -418,6 +418,9 @@ def main():
                     help="Poisson arrival rates (requests/s, comma-separated), one request-loop run each")
     ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "hybrid", "stage", "resident", "auto"),
                     help="batch engine of the headline workload (tuned: measured on this device)")
+    ap.add_argument("--fused-form", default="tuned", choices=("tuned", "auto", "tall", "short"),
+                    help="tile form of the fused kernel on the headline batch (tuned: measured on this device "
+                         "when the engine is tuned; auto: the per-launch rule)")
     ap.add_argument("--grid2048-only", action="store_true",
                     help="only measure the 2048 x 2048 single grid (plain path) and print its JSON")
     ap.add_argument("--slab-only", action="store_true",
@@ -474,6 +477,8 @@ def main():
             tuning = {"engine": tuning, "fused_form": {k: v * 1e6 for k, v in tuner.last.items()}}
             log("tuned fused tile form: %s (%s us/step)" % (form, ", ".join(
                 "%s %.1f" % kv for kv in tuning["fused_form"].items())))
+    if args.fused_form != "tuned":
+        native.fused_form("f32", args.fused_form)
     batch = Batch(members, dim=2, order=5, bc="periodic", dtype="f32", engine=engine, device=dev)
     stream = torch.cuda.current_stream()
     if W > 0:
@@ -512,7 +517,7 @@ def main():
         valu_per_point = read_profile_value(prof, "valu_lane_insts_per_point_step")
         roofline = {"bound": "valu", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                     "frac": achieved / HBM_PEAK, "traffic": traffic,
-                    "kernel": "k_fused_step_2d<float,5,periodic,dynamic>", "kernel_points": fused_points,
+                    "kernel": "k_fused_pairs_2d<periodic,dynamic> (float, order 5)", "kernel_points": fused_points,
                     "algorithmic_bytes_per_launch": bytes_per_launch, "avg_launch_us": launch_s * 1e6,
                     "note": "achieved/peak/frac: the HBM roofline of the algorithmic bytes (secondary); the "
                             "kernel's limiter is its vector ALU work (valu)"}

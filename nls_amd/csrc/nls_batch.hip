@@ -208,6 +208,10 @@ template <typename T> struct FusedArgs {
 //    which outweighs the larger halo when the launch runs more than two
 //    rounds.
 //    With reflecting boundaries this form would spill registers.
+// (A 64 x 64 single-precision form, 1.20 region points per output point
+// instead of 1.31 but 110 KB of LDS and so one workgroup per CU, measured
+// 111 us against 81 us per step on the sweep512 batch: with one workgroup a
+// CU idles at every stage barrier.)
 enum { FORM_TALL = 0, FORM_SHORT = 1, FORM_HALF = 2 };
 template <typename T, int ORDER, int FORM> struct FusedTile;
 template <int ORDER, int FORM> struct FusedTile<float, ORDER, FORM> {
@@ -539,6 +543,16 @@ k_fused_step_2d(const FusedArgs<T> f) {
 #ifndef NLS_FUSED_PAIRS
 #define NLS_FUSED_PAIRS 1
 #endif
+// LDS row stride of the pair kernel's region: W + 2.  In ring order a
+// 16-lane group of the side bands spans four row pairs; a row pair is
+// 2 WS complex values = 4 WS banks apart, which at W + 2 = 82 puts the four
+// in distinct banks (at W + 4 every second one collides: a model of the
+// kernel's reads and writes gives 0.07 against 0.40 conflict cycles per LDS
+// instruction; sweep512 on the hybrid engine 81.7 -> 81.4 us per step).
+#ifndef NLS_PAIR_WS_PAD
+#define NLS_PAIR_WS_PAD 2
+#endif
+constexpr int pair_lds_stride(int W) { return W + NLS_PAIR_WS_PAD; }
 
 template <int W, int HH, int WS> struct PairMap {
     unsigned v[W * HH / 2];
@@ -555,9 +569,9 @@ template <int W, int HH, int WS> struct PairMap {
         }
     }
 };
-static_assert(PairMap<80, 48, 84>().v[0] == (8 * 84 + 8 | 8 << 16 | 8 << 24), "pair order starts at the output tile");
-static_assert(PairMap<80, 48, 84>().v[1024] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the tile");
-static_assert(PairMap<80, 32, 84>().v[512] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the short tile");
+static_assert(PairMap<80, 48, 82>().v[0] == (8 * 82 + 8 | 8 << 16 | 8 << 24), "pair order starts at the output tile");
+static_assert(PairMap<80, 48, 82>().v[1024] == (6 * 82 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the tile");
+static_assert(PairMap<80, 32, 82>().v[512] == (6 * 82 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the short tile");
 template <int W, int HH, int WS> __device__ const PairMap<W, HH, WS> pair_map{};
 
 template <int BC, bool DYN, int FORM>
@@ -568,7 +582,7 @@ __global__ void __launch_bounds__(1024, 8) k_fused_pairs_2d(const FusedArgs<floa
     constexpr int R = 2, H = 4 * R;
     constexpr int TX = 64, TY = FT::TY, NT = FT::NT;
     constexpr int W = TX + 2 * H, HH = TY + 2 * H, NPP = W * HH / 2;
-    constexpr int WS = fused_lds_stride(W, R);
+    constexpr int WS = pair_lds_stride(W);
     constexpr int J = (NPP + NT - 1) / NT;          // pairs per thread
     constexpr int NOUT = TX * TY / 2;              // output pairs: the first NOUT entries
     static_assert(TY % 2 == 0 && HH % 2 == 0 && W < 256 && HH < 256, "pair tile shape");
