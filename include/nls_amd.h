@@ -273,7 +273,11 @@ int nls_observables(const nls_problem *p, const void *psi, const void *nres, dou
  * reservoir, else c12 P / (c13 + c14 |psi|^2)), out[0] = their sum; dV as
  * nls_observables (2*pi*r*dr with the weight r on the radial grid).  The
  * reference defines no energy; its chemical potential is the same operator's
- * expectation, Re mu * norm = out[1] + 2 out[2] + out[3].  Blocks until done. */
+ * expectation, Re mu * norm = out[1] + 2 out[2] + out[3].  For a split-step
+ * (SSF) problem the kinetic term also uses the order-p finite-difference
+ * Laplacian, not the spectral operator the scheme evolves: the two agree to
+ * the stencil's truncation error on a resolved state and differ where the
+ * state has content near the grid's Nyquist frequency.  Blocks until done. */
 int nls_energy(const nls_problem *p, const void *pumping, const void *psi, const void *nres, double *out,
                void *stream);
 
