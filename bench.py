@@ -377,13 +377,16 @@ def plain_f64(members, dev, steps=SWEEP_STEPS):
 
 def serving(rates, capacity_ref=None):
     """Continuous batching: Poisson arrivals at each rate over a window of
-    about 200 requests, the request mix of nls_amd.serving.make_request with
-    inputs given by parameters and generated on the device at admission;
-    latency is arrival to result on the host.  Beside the percentiles: the
-    steady-state rate (work of the blocks run while requests keep arriving,
-    the drain after the last arrival left out), the offered load, and the
-    saturation rate: the highest steady-state rate divided by the mean work
-    of a request."""
+    at least 240 expected requests and 3 s, the request mix of
+    nls_amd.serving.make_request with inputs given by parameters and
+    generated on the device at admission; latency is arrival to result on the
+    host.  Beside the percentiles: the steady-state rate (work of the blocks
+    run while requests keep arriving, the ramp-up and the drain after the
+    last arrival left out) and the offered load.  Then the saturated
+    capacity: 400 requests arriving at once, the rate until half their work
+    is done, beside the batch engine's rate on the same grids; the
+    saturation rate is that capacity divided by the mean work of a
+    request."""
     from nls_amd import serving as S
     runs = []
     for rate in rates:
@@ -400,10 +403,18 @@ def serving(rates, capacity_ref=None):
         "with the tail)" % (sat["requests"], cap / 1e9, sat["grid_point_steps_per_s_incl_tail"] / 1e9))
     out = {"runs": runs, "saturated": sat, "mean_request_grid_point_steps": mean_work,
            "capacity_grid_point_steps_per_s": cap, "saturation_rate_req_s": cap / mean_work}
+    same = sat.get("batch_engine_same_grids_grid_point_steps_per_s")
+    if same:
+        # the batch engine on the saturated run's own grids (every pool full,
+        # no admission, budgets, observables or downloads)
+        out["capacity_vs_batch_engine_same_grids"] = cap / same
+        log("batch engine on the same 400 grids: %.1f G grid-point-steps/s; serving reaches %.0f%% of it" % (
+            same / 1e9, 100 * cap / same))
     if capacity_ref:
-        out["capacity_vs_batch_engine"] = cap / capacity_ref
+        out["capacity_vs_batch_engine_sweep512"] = cap / capacity_ref
     log("serving capacity %.1f G grid-point-steps/s = %.0f requests/s of the mix%s" % (
-        cap / 1e9, cap / mean_work, "" if not capacity_ref else " (%.0f%% of the batch engine)" % (100 * cap / capacity_ref)))
+        cap / 1e9, cap / mean_work,
+        "" if not capacity_ref else " (%.0f%% of the batch engine on sweep512)" % (100 * cap / capacity_ref)))
     return out
 
 

@@ -161,10 +161,12 @@ template <> __device__ __forceinline__ double buffer_load_r<double>(__amdgpu_buf
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
 }
 
-// The fused kernel addresses a member's arrays with 32-bit byte offsets: it
-// holds members (and row slabs) whose complex array is smaller than 4 GiB.
+// Grids (members, row slabs) the fused kernel holds.  In single precision it
+// addresses a member's arrays with 32-bit byte offsets from buffer
+// resources: complex arrays smaller than 4 GiB.  In double precision it takes
+// 64-bit addresses from a 32-bit point index: fewer than 2^32 points.
 bool fused_holds(long long points, int dtype) {
-    return points * (dtype == NLS_F64 ? 16LL : 8LL) <= 0xffffffffLL;
+    return dtype == NLS_F64 ? points <= 0xffffffffLL : points * 8LL <= 0xffffffffLL;
 }
 
 template <typename T> struct FusedArgs {
@@ -297,8 +299,8 @@ k_fused_step_2d(const FusedArgs<T> f) {
     C *uo = f.uo + off;
     T *uno = DYN ? f.uno + off : nullptr;
     const int tid = threadIdx.x;
-    // the region loads take 32-bit byte offsets from buffer resources over
-    // the member's arrays (fused_holds: under 4 GiB)
+    // single precision: the region loads take 32-bit byte offsets from
+    // buffer resources over the member's arrays (fused_holds: under 4 GiB)
     const unsigned cbytes = (unsigned)((unsigned long long)(BC == BC_SLAB ? f.slab_end + H : n) * n * sizeof(C));
     const __amdgpu_buffer_rsrc_t ru = buffer_of(u, cbytes), rp = buffer_of(P, cbytes / 2),
                                  rn = buffer_of(DYN ? un : P, cbytes / 2);
@@ -397,7 +399,7 @@ k_fused_step_2d(const FusedArgs<T> f) {
                 ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
             }
             if (ok) {
-                const unsigned gi = (unsigned)(gy * n + gx);
+                const unsigned gi = grid_index(gy, n, gx);
                 inside |= 1ull << j;
                 if constexpr (sizeof(T) == 4) {
                     uu[j] = buffer_load_c<T>(ru, gi * (unsigned)sizeof(C));
@@ -679,7 +681,7 @@ __global__ void __launch_bounds__(1024, 8) k_fused_pairs_2d(const FusedArgs<floa
                 else if (BC == BC_SLAB) ok = gy < ylim + H;
                 else ok = gx >= 0 && gx < n && gy >= 0 && gy < n;
                 if (ok) {
-                    const unsigned gi = (unsigned)(gy * n + gx);
+                    const unsigned gi = grid_index(gy, n, gx);
                     inside |= 1ull << (2 * j + q);
                     uu[j][q] = buffer_load_c<T>(ru, gi * (unsigned)sizeof(C));
                     cp[j][q] = buffer_load_r<T>(rp, gi * (unsigned)sizeof(T));
@@ -1223,8 +1225,8 @@ static int fused_slab_run(const nls_problem *p, int rows, const T *P, const cplx
 int fused_slab_step(const nls_problem *p, int rows, const void *P, const void *psi_in, const void *nres_in,
                     void *psi_out, void *nres_out, hipStream_t s) {
     if (!fused_holds((long long)(rows + 4 * (p->order - 1)) * p->n, p->dtype))
-        return fail(NLS_EUNSUPPORTED, "a slab of %d rows x %d points is too large for the fused kernel (complex "
-                    "array of 4 GiB or more)", rows, p->n);
+        return fail(NLS_EUNSUPPORTED, "a slab of %d rows x %d points is too large for the fused kernel (single "
+                    "precision: complex array of 4 GiB or more; double: 2^32 points or more)", rows, p->n);
     if (p->dtype == NLS_F64)
         return fused_slab_run<double>(p, rows, (const double *)P, (const cplx_t<double> *)psi_in,
                                       (const double *)nres_in, (cplx_t<double> *)psi_out, (double *)nres_out, s);
@@ -1288,8 +1290,9 @@ int resident_advance(const nls_problem *p, int iters, const void *pumping, void 
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
                        hipStream_t s) {
     if (!fused_holds((long long)p->n * p->n, p->dtype))
-        return fail(NLS_EUNSUPPORTED, "an %d x %d grid is too large for the fused kernel (complex array of 4 GiB or "
-                    "more); the per-stage engine holds it", p->n, p->n);
+        return fail(NLS_EUNSUPPORTED, "an %d x %d grid is too large for the fused kernel (single precision: "
+                    "complex array of 4 GiB or more; double: 2^32 points or more); the per-stage engine holds it",
+                    p->n, p->n);
     if (p->dtype == NLS_F64)
         return fused_grid_run<double>(p, iters, (const double *)pumping, (double2 *)psi, (double *)nres, (char *)ws, s);
     return fused_grid_run<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);

@@ -138,6 +138,15 @@ __device__ __forceinline__ int wrap_near3(int g, int n) {
     return (int)min(min(a, b), c);
 }
 
+// Linear index gy * n + gx of a point of an n x n grid whose index fits 32
+// bits (so gy, n < 2^24): one 24-bit multiply-add.  Written as a 32-bit
+// product the compiler emits a 64-bit multiply-add whose high addend half
+// it may take from any register -- in the fused kernels, one a ring-table
+// load is still filling, so the first field loads waited for that load.
+__device__ __forceinline__ unsigned grid_index(int gy, int n, int gx) {
+    return __umul24((unsigned)gy, (unsigned)n) + (unsigned)gx;
+}
+
 // Periodic index wrap for -n <= g < 2n.
 __device__ __forceinline__ int wrap_near(int g, int n) {
     g += g < 0 ? n : 0;

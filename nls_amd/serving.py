@@ -513,14 +513,17 @@ def mean_request_work(samples=20000, seed=12345, step_range=(1e3, 1e5)):
     return tot / samples
 
 
-def window_for(rate, min_requests=200, min_window=1.0):
-    """Arrival window (s) holding about min_requests requests at `rate`."""
+def window_for(rate, min_requests=240, min_window=3.0):
+    """Arrival window (s) at `rate`: about min_requests expected arrivals
+    (so a Poisson draw holds at least ~200) and at least min_window seconds,
+    so that the steady-state part [0.2 W, W] of an overloaded run is past the
+    ramp-up."""
     return max(min_window, min_requests / float(rate))
 
 
 def serve(rate, window=None, budget_gb=16, seed=0, step_range=(1e3, 1e5), device_inputs=True):
     """One run of the request loop: Poisson arrivals at `rate` over `window`
-    seconds (default: about 200 requests), the serving mix; returns the
+    seconds (default window_for(rate)), the serving mix; returns the
     summary."""
     import torch
     window = window or window_for(rate)
@@ -537,12 +540,14 @@ def serve(rate, window=None, budget_gb=16, seed=0, step_range=(1e3, 1e5), device
     return s
 
 
-def capacity(count=400, budget_gb=16, seed=7, step_range=(1e3, 1e5), device_inputs=True):
+def capacity(count=400, budget_gb=16, seed=7, step_range=(1e3, 1e5), device_inputs=True, reference=True):
     """Saturated throughput of the request loop: `count` requests of the
     serving mix all arriving at once, so every pool runs as full as the HBM
     budget lets it.  The rate is the work of the blocks collected until half
     of all the work is done, per second: the time when the loop is loaded,
-    before the tail in which only the longest requests remain."""
+    before the tail in which only the longest requests remain.  With
+    `reference`, beside it the batch engine's rate on the same requests'
+    grids (batch_rate)."""
     import torch
     rng = np.random.default_rng(seed)
     reqs = [make_request(i, 0.0, rng, step_range, device_inputs=device_inputs) for i in range(count)]
@@ -559,9 +564,68 @@ def capacity(count=400, budget_gb=16, seed=7, step_range=(1e3, 1e5), device_inpu
         if acc >= total / 2:
             t_half = t
             break
-    return {"requests": len(done), "failed": sum(r.error is not None for r in done), "work": total,
-            "wall_s": wall, "half_work_s": t_half, "grid_point_steps_per_s": (total / 2) / t_half,
-            "grid_point_steps_per_s_incl_tail": total / wall}
+    out = {"requests": len(done), "failed": sum(r.error is not None for r in done), "work": total,
+           "wall_s": wall, "half_work_s": t_half, "grid_point_steps_per_s": (total / 2) / t_half,
+           "grid_point_steps_per_s_incl_tail": total / wall}
+    if reference:
+        out["batch_engine_same_grids_grid_point_steps_per_s"] = batch_rate(reqs)
+    return out
+
+
+def batch_rate(reqs, steps=256, dtype="f32", warm=True):
+    """Grid-point-steps/s of the batch engine on the grids of `reqs`: the
+    requests of each pool as one batch on the pool's engine (1-D and 2-D up
+    to 64^2 resident, larger 2-D fused), every member advancing `steps`
+    steps, the three batches on streams of their own at once -- the request
+    loop's pools at full occupancy without admission, budgets, observables or
+    result downloads.  The inputs are a Gaussian pumping and state on each
+    request's grid (the rate does not depend on them).  This is the ceiling
+    the loop's saturated rate is measured against."""
+    import torch
+    from .batch import Batch
+
+    class _M(object):
+        pass
+
+    groups = {}
+    for r in reqs:
+        key = Engine.group_of(r)
+        m = _M()
+        m.n, m.dx, m.dt, m.coeffs = r.n, r.dx, r.dt, np.asarray(r.coeffs, np.float64)
+        if r.dim == 1:
+            x = (np.arange(r.n) - r.n / 2.0) * r.dx
+            rr = np.abs(x)
+        else:
+            x = (np.arange(r.n) - r.n / 2.0) * r.dx
+            rr = np.sqrt(x[None, :] ** 2 + x[:, None] ** 2)
+        m.P = 2.0 * np.exp(-rr ** 2 / 8.0)
+        m.psi0 = (0.1 * np.exp(-rr ** 2 / 4.0)).astype(np.complex128)
+        m.n0 = np.zeros(rr.shape)
+        groups.setdefault(key, []).append(m)
+    engines = {"line": (1, "resident"), "small": (2, "resident"), "large": (2, "fused")}
+    batches, streams, work = [], [], 0
+    for key, members in groups.items():
+        dim, eng = engines[key]
+        batches.append(Batch(members, dim=dim, order=5, bc="periodic", dtype=dtype, engine=eng))
+        streams.append(torch.cuda.Stream())
+        work += sum(m.n if dim == 1 else m.n * m.n for m in members) * steps
+    for b in batches:
+        b.prepare(steps)
+    if warm:
+        for b, s in zip(batches, streams):
+            b.advance(steps, stream=s)
+    torch.cuda.synchronize()
+    for b in batches:
+        b.reset()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for b, s in zip(batches, streams):
+        b.advance(steps, stream=s)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    for b in batches:
+        b.close()
+    return work / wall
 
 
 def sweep(rates=(20, 50, 100, 200, 500), budget_gb=16, seed=0, step_range=(1e3, 1e5), log=print):

@@ -234,6 +234,35 @@ def test_square2048_f64_first_steps(gpu, oracle, engine):
     assert rel(nres32, onres) <= TOL32
 
 
+def test_f64_fused_holds_grids_past_4_gib(gpu):
+    """Double precision keeps 64-bit addresses in the fused kernel, so it
+    holds a 16384^2 grid (its complex array is 4 GiB; single precision, with
+    32-bit buffer offsets, hands such grids to the per-stage engine): two
+    steps on device arrays equal the per-stage engine's to rounding."""
+    import torch
+    n, dx = 16384, 0.01
+    x = (torch.arange(n, dtype=torch.float64, device="cuda") - n / 2) * dx
+    r2 = x[None, :] ** 2 + x[:, None] ** 2
+    P = 2.0 * torch.exp(-r2 / 6.0)
+    psi0 = (0.3 * torch.exp(-r2 / 4.0)).to(torch.complex128)
+    n0 = 0.2 * torch.exp(-r2 / 3.0)
+    del r2
+    c = configs.model_coeffs(dynamic=True)
+    out = {}
+    for engine in ("fused", "stage"):
+        d = native.ProblemDesc.make(2, n, 5, 1e-5, dx, c, bc="periodic", dtype="f64", engine=engine)
+        psi, nres = psi0.clone(), n0.clone()
+        ws = torch.empty(max(1, native.workspace_bytes(d)), dtype=torch.uint8, device="cuda")
+        native.advance(d, 2, P, psi, nres, ws)
+        torch.cuda.synchronize()
+        del ws
+        out[engine] = (psi, nres)
+    for a, b in zip(out["fused"], out["stage"]):
+        err = float(torch.linalg.norm(a - b) / torch.linalg.norm(b))
+        assert err <= 1e-12, err
+    assert float(torch.linalg.norm(out["fused"][0] - psi0)) > 0
+
+
 @pytest.mark.parametrize("dim", (1, 2))
 def test_unstable_dt_is_reported(gpu, dim):
     """A dt far above the RK4 stability limit (|lambda| dt <= 2 sqrt 2) blows

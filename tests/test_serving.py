@@ -157,3 +157,14 @@ def test_blown_up_request_is_evicted_and_others_unchanged(gpu):
         assert done[rid].error is None
         np.testing.assert_array_equal(done[rid].psi, ref[rid].psi)
         np.testing.assert_array_equal(done[rid].nres, ref[rid].nres)
+
+
+@pytest.mark.gpu
+def test_batch_rate_runs_every_pool(gpu):
+    """The batch engine's rate on a set of requests' grids (the reference
+    the loop's saturated rate is reported against) runs the three pools'
+    batches and counts their work."""
+    reqs = [_request_2d(0, 128, 100, seed=21), _request_2d(1, 48, 100, seed=22), _request_1d(2, 1024, 100, seed=23),
+            _request_2d(3, 256, 100, seed=24)]
+    rate = serving.batch_rate(reqs, steps=32)
+    assert np.isfinite(rate) and rate > 1e6
