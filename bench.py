@@ -124,6 +124,27 @@ def cpu_baseline(members, steps):
     out["oracle_all_cores"] = {"value": work / t, "unit": "grid-point-steps/s", "cores": cores, "kind": "port",
                                "sample": sample + "; C restatement, FP32, periodic, reservoir equation, "
                                                   "%d threads" % cores}
+    # the C restatement in both precisions, on one thread and on every core:
+    # median of 3 runs over a smaller sample of the same members
+    few = members[:4]
+    few_steps = max(1, steps // 2)
+    few_work = sum(m.points for m in few) * few_steps
+    table = {}
+    for name, dt in (("f32", np.float32), ("f64", np.float64)):
+        table[name] = {}
+        for threads in (1, cores):
+            O.set_threads(threads)
+            rates = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                for m in few:
+                    O.rk4(2, m.n, 5, 1, m.dt, m.dx, few_steps, m.P, m.coeffs, m.psi0, m.n0, dtype=dt)
+                rates.append(few_work / (time.perf_counter() - t0))
+            table[name][str(threads)] = float(np.median(rates))
+    out["oracle_threads"] = {"unit": "grid-point-steps/s", "kind": "port", "statistic": "median of 3 runs",
+                             "sample": "sweep512 members 0-%d, %d RK4 steps each; C restatement, periodic, "
+                                       "reservoir equation" % (len(few) - 1, few_steps),
+                             "cores": cores, "rates": table}
     return out
 
 
