@@ -454,7 +454,7 @@ def main():
                     help="Poisson arrival rates (requests/s, comma-separated), one request-loop run each")
     ap.add_argument("--engine", default="tuned", choices=("tuned", "fused", "hybrid", "stage", "resident", "auto"),
                     help="batch engine of the headline workload (tuned: measured on this device)")
-    ap.add_argument("--fused-form", default="tuned", choices=("tuned", "auto", "tall", "short"),
+    ap.add_argument("--fused-form", default="tuned", choices=("tuned", "auto", "tall", "short", "tall512"),
                     help="tile form of the fused kernel on the headline batch (tuned: measured on this device "
                          "when the engine is tuned; auto: the per-launch rule)")
     ap.add_argument("--grid2048-only", action="store_true",

@@ -244,7 +244,9 @@ int nls_tune_stage_tile(int dtype, int rows);
  * the tile counts against the compute units (the default), 1 = 64 x 32 tiles
  * of 1024 threads (64 x 16 at order 7), 2 = 64 x 16 tiles of 1024 threads,
  * 3 = 64 x 16 tiles of 512 threads with one stage buffer (double precision
- * on periodic grids; elsewhere it runs as 1); form = -1 only queries.
+ * on periodic grids; elsewhere it runs as 1), 4 = 64 x 32 tiles of 512
+ * threads (single precision, order 5, periodic grids; elsewhere it runs as
+ * 1); form = -1 only queries.
  * Returns the setting in effect or a negative NLS_E* code.  Process-wide; a
  * batch takes the form when it is created or updated.  Set by the
  * autotuner.  No counterpart in the reference. */

@@ -170,7 +170,9 @@ class Tuner(object):
     def form_candidates(dtype, bc, order):
         if order == 7:
             return ["tall"]                        # one form at order 7
-        return ["tall", "short"] + (["half"] if dtype in ("f64", 1) and bc in ("periodic", 1) else [])
+        periodic = bc in ("periodic", 1)
+        return (["tall", "short"] + (["half"] if dtype in ("f64", 1) and periodic else []) +
+                (["tall512"] if dtype in ("f32", 0) and periodic and order == 5 else []))
 
     def _form_choice(self, key, dtype, cands, run, apply):
         prev = native.fused_form(dtype) if self._timer is None else None

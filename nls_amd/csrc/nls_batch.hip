@@ -210,18 +210,24 @@ template <typename T> struct FusedArgs {
 //    which outweighs the larger halo when the launch runs more than two
 //    rounds.
 //    With reflecting boundaries this form would spill registers.
+//  - FORM_TALL512 (single precision, order 5, periodic: the pair kernel):
+//    64 x 32 tiles of 512 threads, four pairs a thread.  A stage's pairs then
+//    fill 87% of the lanes of its passes instead of 75% (the stages step
+//    1672, 1440, 1224 and 1024 pairs), at half the waves per SIMD; chosen by
+//    the autotuner (batches of all-fused 2-D members: 92.8 -> 90.8 us on
+//    sweep512; a lone 2048^2 grid: 33.1 -> 35.2 us).
 // (A 64 x 64 single-precision form, 1.20 region points per output point
 // instead of 1.31 but 110 KB of LDS and so one workgroup per CU, measured
 // 111 us against 81 us per step on the sweep512 batch: with one workgroup a
 // CU idles at every stage barrier.)
-enum { FORM_TALL = 0, FORM_SHORT = 1, FORM_HALF = 2 };
+enum { FORM_TALL = 0, FORM_SHORT = 1, FORM_HALF = 2, FORM_TALL512 = 3 };
 template <typename T, int ORDER, int FORM> struct FusedTile;
 template <int ORDER, int FORM> struct FusedTile<float, ORDER, FORM> {
     static_assert(FORM != FORM_HALF, "single precision has no half-size form");
     static constexpr int TY = ORDER == 7 || FORM == FORM_SHORT ? 16 : 32;
-    static constexpr int NT = 1024;
+    static constexpr int NT = FORM == FORM_TALL512 ? 512 : 1024;
     static constexpr bool DB = true;
-    static constexpr int WAVES = 8;
+    static constexpr int WAVES = FORM == FORM_TALL512 ? 4 : 8;
 };
 template <int ORDER, int FORM> struct FusedTile<double, ORDER, FORM> {
     static constexpr bool HALF = FORM == FORM_HALF;
@@ -234,7 +240,9 @@ template <int ORDER, int FORM> struct FusedTile<double, ORDER, FORM> {
 // ring order's side bands (2R points per row over consecutive rows) fall in
 // distinct banks.
 constexpr int fused_lds_stride(int W, int R) { return W + (R == 1 ? 2 : R == 2 ? 4 : 14); }
-static int fused_ty(int order, int form) { return order == 7 || form != FORM_TALL ? 16 : 32; }
+static int fused_ty(int order, int form) {
+    return order == 7 || (form != FORM_TALL && form != FORM_TALL512) ? 16 : 32;
+}
 
 // Region points in ring order, built at compile time: entry k holds the
 // point's LDS index y WS + x in bits 0..15, x in bits 16..23, y in 24..31.
@@ -577,7 +585,8 @@ static_assert(PairMap<80, 32, 82>().v[512] == (6 * 82 + 6 | 6 << 16 | 6 << 24), 
 template <int W, int HH, int WS> __device__ const PairMap<W, HH, WS> pair_map{};
 
 template <int BC, bool DYN, int FORM>
-__global__ void __launch_bounds__(1024, 8) k_fused_pairs_2d(const FusedArgs<float> f) {
+__global__ void __launch_bounds__((FusedTile<float, 5, FORM>::NT), (FusedTile<float, 5, FORM>::WAVES))
+k_fused_pairs_2d(const FusedArgs<float> f) {
     using T = float;
     using C = float2;
     using FT = FusedTile<float, 5, FORM>;
@@ -1009,8 +1018,8 @@ static void launch_batch_stage(int dim, int order, int bc, bool dyn, const Batch
 
 // One fused step launch in the given form; forms a configuration does not
 // build (order 7 has one form; FORM_HALF is double precision without
-// reflecting boundaries) fall back to FORM_TALL, which fused_form never asks
-// otherwise.
+// reflecting boundaries; FORM_TALL512 is the single-precision pair kernel)
+// fall back to FORM_TALL, which fused_form never asks otherwise.
 template <typename T, int O, int B, bool D>
 static void launch_fused_form(int form, const FusedArgs<T> &f, hipStream_t s) {
     const dim3 grid(8 * f.chunk);
@@ -1019,6 +1028,8 @@ static void launch_fused_form(int form, const FusedArgs<T> &f, hipStream_t s) {
     if constexpr (sizeof(T) == 4 && O == 5 && B == BC_PERIODIC && NLS_FUSED_PAIRS) {
         if (form == FORM_SHORT)
             hipLaunchKernelGGL((k_fused_pairs_2d<B, D, FORM_SHORT>), grid, dim3(1024), 0, s, f);
+        else if (form == FORM_TALL512)
+            hipLaunchKernelGGL((k_fused_pairs_2d<B, D, FORM_TALL512>), grid, dim3(512), 0, s, f);
         else
             hipLaunchKernelGGL((k_fused_pairs_2d<B, D, FORM_TALL>), grid, dim3(1024), 0, s, f);
         return;
@@ -1084,7 +1095,7 @@ static int device_cus() {
 static std::atomic<int> g_fused_form[2] = {{-1}, {-1}};
 
 int set_fused_form(int dtype, int form) {
-    if (form < -1 || form > FORM_HALF) return -1;
+    if (form < -1 || form > FORM_TALL512) return -1;
     g_fused_form[dtype == NLS_F64 ? 1 : 0].store(form);
     return 0;
 }
@@ -1095,6 +1106,8 @@ static int fused_form(int dtype, int order, int bc, long long tiles16, long long
     if (forced >= 0) {
         // forms a configuration does not build run tall (launch_fused_form)
         if (order == 7 || (forced == FORM_HALF && (dtype != NLS_F64 || bc == BC_REF))) return FORM_TALL;
+        if (forced == FORM_TALL512 && !(dtype == NLS_F32 && order == 5 && bc == BC_PERIODIC && NLS_FUSED_PAIRS))
+            return FORM_TALL;
         return forced;
     }
     if (order == 7) return FORM_TALL;

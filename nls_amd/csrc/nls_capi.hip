@@ -657,9 +657,10 @@ int nls_runge_kutta_2d(float dt, float t0, const void *u0, int n, const float *b
 
 int nls_tune_fused_form(int dtype, int form) {
     if (dtype != NLS_F32 && dtype != NLS_F64) return fail(NLS_EINVAL, "bad dtype");
-    if (form == -1) return get_fused_form(dtype) + 1;     // internal -1 (per launch) .. 2 -> 0 .. 3
-    if (form < 0 || form > 3 || set_fused_form(dtype, form - 1))
-        return fail(NLS_EINVAL, "form must be 0 (per launch), 1 (tall), 2 (short) or 3 (half) (got %d)", form);
+    if (form == -1) return get_fused_form(dtype) + 1;     // internal -1 (per launch) .. 3 -> 0 .. 4
+    if (form < 0 || form > 4 || set_fused_form(dtype, form - 1))
+        return fail(NLS_EINVAL, "form must be 0 (per launch), 1 (tall), 2 (short), 3 (half) or 4 (tall, 512 "
+                    "threads) (got %d)", form);
     return form;
 }
 

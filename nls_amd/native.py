@@ -499,12 +499,13 @@ def stage_tile(dtype, rows=0):
     return r
 
 
-FUSED_FORMS = {"auto": 0, "tall": 1, "short": 2, "half": 3}
+FUSED_FORMS = {"auto": 0, "tall": 1, "short": 2, "half": 3, "tall512": 4}
 
 
 def fused_form(dtype, form=None):
     """Get (form=None) or set the fused kernel's tile form for dtype:
-    "auto" (per launch), "tall", "short" or "half" (nls_tune_fused_form)."""
+    "auto" (per launch), "tall", "short", "half" or "tall512"
+    (nls_tune_fused_form)."""
     code = -1 if form is None else FUSED_FORMS[form]
     r = load().nls_tune_fused_form(DTYPE[dtype] if isinstance(dtype, str) else int(dtype), code)
     if r < 0:

@@ -72,8 +72,10 @@ def test_stage_tile_and_batch_choices():
 
 def test_fused_form_candidates_and_choice():
     T = autotune.Tuner(path=None, device_name="dev",
-                       timer=lambda cand, _s: {"tall": 2.0, "short": 3.0, "half": 1.0}[cand[1]])
-    assert T.form_candidates("f32", "periodic", 5) == ["tall", "short"]
+                       timer=lambda cand, _s: {"tall": 2.0, "short": 3.0, "half": 1.0, "tall512": 2.5}[cand[1]])
+    assert T.form_candidates("f32", "periodic", 5) == ["tall", "short", "tall512"]
+    assert T.form_candidates("f32", "periodic", 3) == ["tall", "short"]
+    assert T.form_candidates("f32", "ref", 5) == ["tall", "short"]
     assert T.form_candidates("f64", "periodic", 5) == ["tall", "short", "half"]
     assert T.form_candidates("f64", "ref", 3) == ["tall", "short"]
     assert T.form_candidates("f64", "periodic", 7) == ["tall"]
@@ -94,9 +96,10 @@ def test_tuned_fused_form_batch_matches_each_form(gpu):
     T = autotune.Tuner(path=None, steps=16, reps=1)
     try:
         best = T.fused_form(ms, dtype="f32", engine="fused")
-        assert best in ("tall", "short") and set(T.last) == {"tall", "short"} and native.fused_form("f32") == best
+        assert best in ("tall", "short", "tall512") and set(T.last) == {"tall", "short", "tall512"}
+        assert native.fused_form("f32") == best
         out = {}
-        for form, ty in (("tall", 32), ("short", 16)):
+        for form, ty in (("tall", 32), ("short", 16), ("tall512", 32)):
             native.fused_form("f32", form)
             b = Batch(ms, dim=2, order=5, bc="periodic", dtype="f32", engine="fused")
             assert b.tiles == sum(((m.n + 63) // 64) * ((m.n + ty - 1) // ty) for m in ms)
@@ -104,8 +107,42 @@ def test_tuned_fused_form_batch_matches_each_form(gpu):
             out[form] = b.psi.cpu().numpy()
             b.close()
         np.testing.assert_array_equal(out["tall"], out["short"])
+        np.testing.assert_array_equal(out["tall"], out["tall512"])
     finally:
         native.fused_form("f32", "auto")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("n", [40, 72, 130, 256])
+def test_plain_tall512_form_equals_tall(gpu, n):
+    """The 512-thread form of the single-precision periodic pair kernel gives
+    the 1024-thread form's result bit for bit on a plain solve (grids smaller
+    than one tile, not a multiple of it, several tiles); in double precision
+    and with reflecting boundaries it runs as the tall form."""
+    X, Y, h = configs.grid_2d(n, 10.0)
+    r2 = X ** 2 + Y ** 2
+    rng = np.random.default_rng(n)
+    psi0 = 0.3 * np.exp(-r2 / 4.0) + configs.noise(rng, (n, n), 1e-2)
+    P = 2.0 * np.exp(-r2 / 6.0)
+    n0 = 0.2 * np.exp(-r2 / 3.0)
+    try:
+        for dtype, bc, dyn in (("f32", "periodic", True), ("f32", "periodic", False), ("f64", "periodic", True),
+                               ("f32", "ref", True)):
+            c = configs.model_coeffs(dynamic=dyn)
+            out = {}
+            for form in ("tall", "tall512"):
+                native.fused_form(dtype, form)
+                d = native.ProblemDesc.make(2, n, 5, 2e-4, h, c, bc=bc, dtype=dtype, engine="fused")
+                out[form] = native.solve_host(d, 37, P, psi0, n0)
+            for a, b in zip(out["tall"], out["tall512"]):
+                if a is None:                      # no reservoir field without the reservoir equation
+                    assert b is None
+                    continue
+                assert np.all(np.isfinite(a))
+                np.testing.assert_array_equal(a, b)
+    finally:
+        native.fused_form("f32", "auto")
+        native.fused_form("f64", "auto")
 
 
 @pytest.mark.gpu
