@@ -305,10 +305,9 @@ def config_accuracy(dev):
         out[name] = {"relL2_f32_vs_f64": err, "steps": inst.steps, "points": inst.points,
                      "fp64_accumulated": {"norm_f64": obs["f64"]["norm"], "norm_f32": obs["f32"]["norm"],
                                           "energy_f64": obs["f64"]["energy"], "energy_f32": obs["f32"]["energy"]}}
-        if scheme == "ssf":
-            # nls_energy's kinetic term is the order-5 finite-difference one,
-            # not the spectral operator the split-step scheme evolves
-            out[name]["fp64_accumulated"]["energy_kinetic_operator"] = "finite-difference order 5"
+        # nls_energy's kinetic term uses the Laplacian the scheme evolves
+        out[name]["fp64_accumulated"]["energy_kinetic_operator"] = (
+            "spectral" if scheme == "ssf" else "finite-difference order %d" % inst.order)
         log("accuracy %s (%d steps): FP32 vs FP64 relL2 = %.2e; norm %.10g / %.10g, energy %.10g / %.10g "
             "(FP64 / FP32 state, FP64 sums)" % (name, inst.steps, err, obs["f64"]["norm"], obs["f32"]["norm"],
                                                obs["f64"]["energy"], obs["f32"]["energy"]))

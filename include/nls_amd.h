@@ -270,16 +270,18 @@ int nls_observables(const nls_problem *p, const void *psi, const void *nres, dou
 /* Energy of a device state for the conservative part of the right-hand side
  * (nls.f90:637-644: the operator -L + c5|psi|^2 + c6 r), double-precision
  * accumulators in a fixed order: out[1] = kinetic -Re<psi, L psi> dV with the
- * problem's own discrete Laplacian L, out[2] = interaction (c5/2) sum |psi|^4
+ * Laplacian L the problem's scheme evolves, out[2] = interaction (c5/2) sum |psi|^4
  * dV, out[3] = reservoir coupling c6 sum r |psi|^2 dV (r = n for the dynamic
  * reservoir, else c12 P / (c13 + c14 |psi|^2)), out[0] = their sum; dV as
  * nls_observables (2*pi*r*dr with the weight r on the radial grid).  The
  * reference defines no energy; its chemical potential is the same operator's
- * expectation, Re mu * norm = out[1] + 2 out[2] + out[3].  For a split-step
- * (SSF) problem the kinetic term also uses the order-p finite-difference
- * Laplacian, not the spectral operator the scheme evolves: the two agree to
- * the stencil's truncation error on a resolved state and differ where the
- * state has content near the grid's Nyquist frequency.  Blocks until done. */
+ * expectation, Re mu * norm = out[1] + 2 out[2] + out[3] (mu with the
+ * finite-difference operator).  L is the order-p finite-difference stencil
+ * for RK4 problems and the spectral operator for split-step (SSF) problems:
+ * out[1] = (dV / N^d) sum_k |k|^2 |psi_hat(k)|^2 over the grid's
+ * wavenumbers k = 2 pi f / (n dx), f = -n/2 .. n/2 - 1 (taken from the 1-D
+ * transforms of every row and column), which the scheme's linear step
+ * conserves exactly.  Blocks until done. */
 int nls_energy(const nls_problem *p, const void *pumping, const void *psi, const void *nres, double *out,
                void *stream);
 

@@ -37,6 +37,11 @@ int resident_advance(const nls_problem *p, int iters, const void *pumping, void 
 // split-step Fourier scheme (nls_ssf.hip)
 size_t ssf_workspace_bytes(const nls_problem *p);
 int ssf_check(const nls_problem *p);               // size limits of the scheme
+// (1/N^d) sum_k |k|^2 |psi_hat(k)|^2 = -Re<psi, L psi> with the spectral
+// Laplacian of a split-step problem, into dev_sum (device, one double);
+// scratch: ssf_kinetic_scratch_bytes of device memory
+size_t ssf_kinetic_scratch_bytes(const nls_problem *p);
+int ssf_kinetic(const nls_problem *p, const void *psi, void *scratch, double *dev_sum, hipStream_t s);
 int ssf_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
                 hipStream_t s);
 

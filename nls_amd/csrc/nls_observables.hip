@@ -8,7 +8,9 @@
 //   - the energy of the conservative part of the right-hand side,
 //     E = -Re<psi, L psi>_w + (c5/2) sum |psi|^4 w + c6 sum r |psi|^2 w
 //     (times the volume element), whose variation at fixed reservoir r gives
-//     the operator -L + c5|psi|^2 + c6 r of nls.f90:637-644.
+//     the operator -L + c5|psi|^2 + c6 r of nls.f90:637-644; L is the
+//     problem's finite-difference Laplacian, or for a split-step problem the
+//     spectral one the scheme evolves (nls_ssf.hip, ssf_kinetic).
 // Pass 1: each 256-thread workgroup reduces a grid-strided slice in a fixed
 // tree; pass 2: one workgroup sums the per-workgroup partials in order.
 #include <cstring>
@@ -188,8 +190,11 @@ static int energy_dev(const nls_problem *p, const void *P, const void *psi, cons
     nls_problem q = *p;
     q.coeffs[2] = q.coeffs[3] = q.coeffs[4] = q.coeffs[5] = 0.0;
     q.coeffs[10] = 0.0; q.coeffs[11] = 0.0; q.coeffs[12] = 1.0; q.coeffs[13] = 0.0;
+    // the kinetic term with the scheme's own Laplacian: the stencil's for
+    // RK4, the spectral one for the split-step scheme
+    const bool spectral = p->scheme == NLS_SCHEME_SSF;
     void *scratch = nullptr;
-    const size_t fbytes = N * 2 * rs;
+    const size_t fbytes = spectral ? ssf_kinetic_scratch_bytes(p) : N * 2 * rs;
     const size_t red = (size_t)RED_BLOCKS * RED_VALS * sizeof(double) + 2 * RED_VALS * sizeof(double);
     int rc = hip_check(hipMalloc(&scratch, fbytes + red + 512), "energy scratch");
     if (rc) return rc;
@@ -197,17 +202,18 @@ static int energy_dev(const nls_problem *p, const void *P, const void *psi, cons
     void *F = base;
     double *part = (double *)(base + ((fbytes + 255) & ~(size_t)255));
     double *outK = part + RED_BLOCKS * RED_VALS, *outV = outK + RED_VALS;
-    rc = nls_rhs(&q, P, psi, nullptr, F, nullptr, s);
+    rc = spectral ? ssf_kinetic(p, psi, F, outK, s) : nls_rhs(&q, P, psi, nullptr, F, nullptr, s);
     const int radial = (p->dim == 1 && p->bc == NLS_BC_REF) ? 1 : 0;
     const int nb = red_blocks(N);
     if (!rc) {
+        // (the spectral kinetic sum is already in outK[0])
+        if (!spectral && p->dtype == NLS_F64) cdot<double>(N, psi, F, radial, p->dx, part, outK, s);
+        else if (!spectral) cdot<float>(N, psi, F, radial, p->dx, part, outK, s);
         if (p->dtype == NLS_F64) {
-            cdot<double>(N, psi, F, radial, p->dx, part, outK, s);
             hipLaunchKernelGGL((k_energy_partial<double>), dim3(nb), dim3(256), 0, s, N, (const double2 *)psi,
                                (const double *)P, dyn ? (const double *)nres : nullptr, p->coeffs[11], p->coeffs[12],
                                p->coeffs[13], radial, p->dx, part);
         } else {
-            cdot<float>(N, psi, F, radial, p->dx, part, outK, s);
             hipLaunchKernelGGL((k_energy_partial<float>), dim3(nb), dim3(256), 0, s, N, (const float2 *)psi,
                                (const float *)P, dyn ? (const float *)nres : nullptr, p->coeffs[11], p->coeffs[12],
                                p->coeffs[13], radial, p->dx, part);
@@ -222,7 +228,8 @@ static int energy_dev(const nls_problem *p, const void *P, const void *psi, cons
     (void)hipFree(scratch);
     if (rc) return rc;
     const double dV = radial ? 2.0 * M_PI * p->dx : (p->dim == 1 ? p->dx : p->dx * p->dx);
-    out[1] = -hK[1] * dV;                               // -Re<psi, L psi> = -Im<psi, i L psi>
+    // -Re<psi, L psi> = -Im<psi, i L psi>; spectral: (1/n) (sum over lines)
+    out[1] = spectral ? hK[0] / (double)p->n * dV : -hK[1] * dV;
     out[2] = 0.5 * p->coeffs[4] * hV[0] * dV;
     out[3] = p->coeffs[5] * hV[1] * dV;
     out[0] = out[1] + out[2] + out[3];

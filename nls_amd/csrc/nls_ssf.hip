@@ -596,6 +596,178 @@ static int ssf_run(const nls_problem *p, int iters, const S *P, cplx_t<S> *psi, 
     return hip_check(hipGetLastError(), "ssf launch");
 }
 
+// ---------------------------------------------------------------------------
+// Kinetic energy with the scheme's own (spectral) operator:
+//   -Re<psi, L psi> = (1 / N^d) sum_k |k|^2 |psi_hat(k)|^2
+// (psi_hat the unnormalised DFT, N^d grid points).  In 2-D it is taken from
+// 1-D transforms only: by Parseval along the other axis,
+//   sum_{kx,ky} kx^2 |psi_hat|^2 = N sum_rows sum_kx kx^2 |DFT_x(row)|^2
+// and likewise for ky over the columns, so
+//   -Re<psi, L psi> = (1/N) (sum over rows + sum over columns)
+// with no 2-D transform and no intermediate field.  Each workgroup reduces
+// its lines' sum of k^2 |DFT|^2 in a fixed tree into one partial; one
+// workgroup adds the partials in order.  Double precision throughout.
+
+// sum over the workgroup's threads of v, in a fixed order, into *out
+// (thread 0); sh: NTH doubles of LDS, free on entry
+template <int NTH> __device__ __forceinline__ void fixed_block_sum(double *sh, double v, double *out) {
+    const int t = threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+#pragma unroll
+    for (int s = NTH / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    if (t == 0) *out = sh[0];
+}
+
+// k_m^2 |v|^2 for FFT output index m of a length-N transform (frequency m,
+// or m - N past N/2, as the phase table), k_m = k0 * frequency
+__device__ __forceinline__ double k2_abs2(int m, int N, double k0, cplx_t<double> v) {
+    const int f = m < N / 2 ? m : m - N;
+    const double k = k0 * (double)f;
+    return k * k * (v.x * v.x + v.y * v.y);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(ROW_NTH) k_ssf_kin_rows(const cplx_t<S> *psi, int logN, const cplx_t<double> *tw,
+                                                          double k0, double *part) {
+    using C = cplx_t<double>;
+    constexpr int PPT = ROW_PTS / ROW_NTH;
+    __shared__ C buf_mem[lds_padded(ROW_PTS, NLS_SSF_ROW_PAD)];
+    const Lds<double, NLS_SSF_ROW_PAD> buf{buf_mem};
+    const int N = 1 << logN;
+    const int logL = lines_per_wg(logN, ROW_LOG_PTS);
+    const int pts = 1 << (logL + logN);
+    const size_t base = (size_t)blockIdx.x << (logL + logN);
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + ROW_NTH * q;
+        if (idx < pts) buf[idx] = widen<double, S>(psi[base + idx]);
+    }
+    __syncthreads();
+    lds_fft<double, false, false, ROW_PTS, ROW_NTH>(buf, logN, logL, N, 1, tw);
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + ROW_NTH * q;
+        if (idx < pts) v += k2_abs2(idx & (N - 1), N, k0, buf[idx]);
+    }
+    __syncthreads();
+    fixed_block_sum<ROW_NTH>((double *)buf_mem, v, part + blockIdx.x);
+}
+
+template <typename S>
+__global__ void __launch_bounds__(ColShape<double>::NTH) k_ssf_kin_cols(const cplx_t<S> *psi, int logN,
+                                                                         const cplx_t<double> *tw, double k0,
+                                                                         double *part) {
+    using C = cplx_t<double>;
+    constexpr int COL_PTS = ColShape<double>::PTS, COL_NTH = ColShape<double>::NTH;
+    constexpr int PPT = COL_PTS / COL_NTH;
+    __shared__ C buf_mem[lds_padded(COL_PTS, NLS_SSF_COL_PAD)];
+    const Lds<double, NLS_SSF_COL_PAD> buf{buf_mem};
+    const int N = 1 << logN;
+    const int logL = ColShape<double>::LOG_L < logN ? ColShape<double>::LOG_L : logN;
+    const int L = 1 << logL, pts = L << logN;
+    const int col0 = blockIdx.x << logL;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + COL_NTH * q;
+        if (idx < pts) buf[idx] = widen<double, S>(psi[(size_t)(idx >> logL) * N + col0 + (idx & (L - 1))]);
+    }
+    __syncthreads();
+    lds_fft<double, false, true, COL_PTS, COL_NTH>(buf, logN, logL, 1, L, tw);
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + COL_NTH * q;
+        if (idx < pts) v += k2_abs2(idx >> logL, N, k0, buf[idx]);
+    }
+    __syncthreads();
+    fixed_block_sum<COL_NTH>((double *)buf_mem, v, part + blockIdx.x);
+}
+
+// 1-D: one workgroup, the whole line (n <= 4096)
+template <typename S>
+__global__ void __launch_bounds__(LineShape<double>::NTH) k_ssf_kin_line(const cplx_t<S> *psi, int logN,
+                                                                          const cplx_t<double> *tw, double k0,
+                                                                          double *part) {
+    using C = cplx_t<double>;
+    constexpr int LINE_NTH = LineShape<double>::NTH;
+    constexpr int PPT = LINE_PTS / LINE_NTH;
+    __shared__ C buf_mem[lds_padded(LINE_PTS, NLS_SSF_LINE_PAD)];
+    const Lds<double, NLS_SSF_LINE_PAD> buf{buf_mem};
+    const int N = 1 << logN;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + LINE_NTH * q;
+        if (idx < N) buf[idx] = widen<double, S>(psi[idx]);
+    }
+    __syncthreads();
+    lds_fft<double, false, false, LINE_PTS, LINE_NTH>(buf, logN, 0, N, 1, tw);
+    double v = 0.0;
+#pragma unroll
+    for (int q = 0; q < PPT; ++q) {
+        const int idx = threadIdx.x + LINE_NTH * q;
+        if (idx < N) v += k2_abs2(idx, N, k0, buf[idx]);
+    }
+    __syncthreads();
+    fixed_block_sum<LINE_NTH>((double *)buf_mem, v, part);
+}
+
+// partials 0 .. nb-1 added in order by one workgroup of 256 threads: thread
+// t adds t, t + 256, ...; then a fixed tree
+__global__ void __launch_bounds__(256) k_ssf_kin_sum(int nb, const double *part, double *out) {
+    __shared__ double sh[256];
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nb; b += 256) v += part[b];
+    fixed_block_sum<256>(sh, v, out);
+}
+
+size_t ssf_kinetic_scratch_bytes(const nls_problem *p) {
+    // twiddles and phases (the table kernel writes both), the partials, the sum
+    return 2 * align256s((size_t)p->n * 16) + align256s(((size_t)2 * p->n + 1) * sizeof(double));
+}
+
+int ssf_kinetic(const nls_problem *p, const void *psi, void *scratch, double *dev_sum, hipStream_t s) {
+    int rc = ssf_check(p);
+    if (rc) return rc;
+    using C = cplx_t<double>;
+    const int N = p->n, logN = log2_exact(N);
+    char *ws = (char *)scratch;
+    C *tw = (C *)ws, *ph = (C *)(ws + align256s((size_t)N * sizeof(C)));
+    double *part = (double *)(ws + 2 * align256s((size_t)N * sizeof(C)));
+    hipLaunchKernelGGL(k_ssf_tables<double>, dim3((N + 255) / 256), dim3(256), 0, s, N, p->dx, p->dt, 1.0, tw, ph);
+    const double k0 = 2.0 * M_PI / ((double)N * p->dx);
+    const bool f64 = p->dtype == NLS_F64;
+    int nb;
+    if (p->dim == 1) {
+        nb = 1;
+        if (f64) hipLaunchKernelGGL(k_ssf_kin_line<double>, dim3(1), dim3(LineShape<double>::NTH), 0, s,
+                                    (const double2 *)psi, logN, tw, k0, part);
+        else hipLaunchKernelGGL(k_ssf_kin_line<float>, dim3(1), dim3(LineShape<double>::NTH), 0, s,
+                                (const float2 *)psi, logN, tw, k0, part);
+    } else {
+        const int rblocks = N >> lines_per_wg(logN, ROW_LOG_PTS);
+        const int cblocks = N >> (ColShape<double>::LOG_L < logN ? ColShape<double>::LOG_L : logN);
+        nb = rblocks + cblocks;
+        if (f64) {
+            hipLaunchKernelGGL(k_ssf_kin_rows<double>, dim3(rblocks), dim3(ROW_NTH), 0, s, (const double2 *)psi,
+                               logN, tw, k0, part);
+            hipLaunchKernelGGL(k_ssf_kin_cols<double>, dim3(cblocks), dim3(ColShape<double>::NTH), 0, s,
+                               (const double2 *)psi, logN, tw, k0, part + rblocks);
+        } else {
+            hipLaunchKernelGGL(k_ssf_kin_rows<float>, dim3(rblocks), dim3(ROW_NTH), 0, s, (const float2 *)psi,
+                               logN, tw, k0, part);
+            hipLaunchKernelGGL(k_ssf_kin_cols<float>, dim3(cblocks), dim3(ColShape<double>::NTH), 0, s,
+                               (const float2 *)psi, logN, tw, k0, part + rblocks);
+        }
+    }
+    hipLaunchKernelGGL(k_ssf_kin_sum, dim3(1), dim3(256), 0, s, nb, part, dev_sum);
+    return hip_check(hipGetLastError(), "ssf kinetic energy launch");
+}
+
 size_t ssf_workspace_bytes(const nls_problem *p) {
     return 2 * align256s((size_t)p->n * 16);          // twiddles and phases, double precision
 }

@@ -585,9 +585,9 @@ def chemical_potential(desc, pumping, psi, nres=None, stream=None):
 def energy(desc, pumping, psi, nres=None, stream=None):
     """Energy of a device state for the conservative part of the right-hand
     side (nls_energy): {"energy", "kinetic", "interaction", "reservoir"},
-    accumulated in double precision.  The kinetic term uses the problem's
-    order-p finite-difference Laplacian, also for a split-step (SSF) problem,
-    whose scheme evolves the spectral operator instead."""
+    accumulated in double precision.  The kinetic term uses the Laplacian the
+    problem's scheme evolves: the order-p finite-difference stencil for RK4,
+    the spectral operator (sum of |k|^2 |psi_hat|^2) for split-step (SSF)."""
     import torch
     s = stream if stream is not None else torch.cuda.current_stream()
     out = (C.c_double * 4)()

@@ -7,7 +7,12 @@ eigenvalue in closed form; the identity Re(mu) * norm = kinetic +
 2 interaction + reservoir with the chemical potential (nls_chemical_potential,
 an independent path through the full right-hand side) and the norm
 (nls_observables); the oracle's Laplacian for the kinetic term on the
-reference's boundaries (radial 1-D, zero outside the square)."""
+reference's boundaries (radial 1-D, zero outside the square).
+
+Split-step problems take the kinetic term with the spectral operator: pinned
+by plane waves up to the Nyquist frequency (eigenvalue -|k|^2), numpy's
+d-dimensional FFT, its exact conservation by the scheme's linear step and its
+conservation (to the splitting error) by the conservative nonlinear flow."""
 import itertools
 
 import numpy as np
@@ -160,3 +165,106 @@ def test_batch_energy_equals_each_member_alone(gpu, dim, bc, engine, dtype, dyn)
             assert abs(got[i, c] - want[k]) <= tol * (abs(want["kinetic"]) + abs(want["interaction"])
                                                      + abs(want["reservoir"])), (i, k, got[i, c], want[k])
     b.close()
+
+
+# ---- split-step (SSF) problems: the spectral kinetic term ------------------
+
+def ssf_wavenumbers(n, h):
+    return 2 * np.pi * np.fft.fftfreq(n, d=h)
+
+
+@pytest.mark.parametrize("dim,n", [(1, 64), (1, 4096), (2, 16), (2, 256), (2, 2048)])
+@pytest.mark.parametrize("dtype", ["f64", "f32"])
+def test_ssf_kinetic_plane_wave_closed_form(gpu, dim, n, dtype):
+    """A plane wave exp(i k.x) is an eigenvector of the spectral Laplacian
+    with eigenvalue -|k|^2 for every grid wavenumber, the Nyquist one
+    included (where the order-5 stencil's eigenvalue is 4/3 |k|^2 h^2
+    smaller in size): kinetic = |k|^2 * norm.  The interaction and reservoir
+    terms do not depend on the operator."""
+    h, A = 0.125, 0.7 + 0.2j
+    j = np.arange(n)
+    for m in ((3, 5), (n // 2 - 1, 1), (n // 2, n // 2)):
+        if dim == 1:
+            psi = A * np.exp(2j * np.pi * m[0] * j / n)
+            k2 = ssf_wavenumbers(n, h)[m[0] % n] ** 2
+        else:
+            J, K = np.meshgrid(j, j, indexing="ij")
+            psi = A * np.exp(2j * np.pi * (m[0] * J + m[1] * K) / n)
+            k = ssf_wavenumbers(n, h)
+            k2 = k[m[0] % n] ** 2 + k[m[1] % n] ** 2
+        c = coeffs(True)
+        P = np.full(psi.shape, 2.5)
+        n0 = np.full(psi.shape, 0.4)
+        d = native.ProblemDesc.make(dim, n, 5, 1e-3, h, c, bc="periodic", dtype=dtype, scheme="ssf")
+        e = native.energy(d, dev(P, dtype), dev(psi, dtype), dev(n0, dtype))
+        s = abs(np.complex64(A) if dtype == "f32" else A) ** 2
+        norm = s * (n * h) ** dim
+        tol = 1e-12 if dtype == "f64" else 1e-6
+        assert abs(e["kinetic"] - k2 * norm) <= tol * k2 * norm, (m, e["kinetic"], k2 * norm)
+        want_int = 0.5 * c[4] * s * s * (n * h) ** dim
+        assert abs(e["interaction"] - want_int) <= tol * want_int
+
+
+@pytest.mark.parametrize("dim,n", [(1, 512), (2, 64), (2, 512), (2, 2048)])
+def test_ssf_kinetic_matches_numpy_fft(gpu, dim, n):
+    """A smooth state with noise up to the Nyquist frequency: the kinetic
+    term against (dV / N^d) sum |k|^2 |fftn(psi)|^2 from numpy's full
+    d-dimensional transform (the device takes 1-D transforms of the rows
+    and columns only)."""
+    h, psi, P, n0 = smooth_case(dim, n, "periodic", True, seed=n)
+    d = native.ProblemDesc.make(dim, n, 5, 1e-3, h, coeffs(True), bc="periodic", dtype="f64", scheme="ssf")
+    e = native.energy(d, dev(P, "f64"), dev(psi, "f64"), dev(n0, "f64"))
+    k = ssf_wavenumbers(n, h)
+    if dim == 1:
+        k2 = k ** 2
+    else:
+        k2 = k[:, None] ** 2 + k[None, :] ** 2
+    want = float(np.sum(k2 * np.abs(np.fft.fftn(psi)) ** 2)) * h ** dim / psi.size
+    assert abs(e["kinetic"] - want) <= 1e-12 * want, (e["kinetic"], want)
+    # the stencil's kinetic term of the same state is smaller (its eigenvalues
+    # fall below |k|^2 towards the Nyquist frequency, where the noise sits)
+    fd = native.energy(native.ProblemDesc.make(dim, n, 5, 1e-3, h, coeffs(True), bc="periodic", dtype="f64"),
+                       dev(P, "f64"), dev(psi, "f64"), dev(n0, "f64"))
+    assert fd["kinetic"] < e["kinetic"]
+
+
+@pytest.mark.parametrize("dim,n", [(1, 1024), (2, 256)])
+def test_ssf_linear_flow_conserves_spectral_kinetic_energy(gpu, dim, n):
+    """With every pointwise term off (c3 = c4 = c5 = c6 = 0) a split step
+    only multiplies each mode by a unit phase, so the spectral kinetic energy
+    is conserved to rounding over hundreds of steps."""
+    h, psi, P, _ = smooth_case(dim, n, "periodic", False, seed=7)
+    c = np.zeros(23)
+    c[12] = 1.0
+    d = native.ProblemDesc.make(dim, n, 5, 5e-3, h, c, bc="periodic", dtype="f64", scheme="ssf")
+    before = native.energy(d, dev(P, "f64"), dev(psi, "f64"))
+    out, _ = native.solve_host(d, 300, P, psi, None)
+    after = native.energy(d, dev(P, "f64"), dev(np.asarray(out), "f64"))
+    assert abs(after["kinetic"] - before["kinetic"]) <= 1e-12 * before["kinetic"]
+
+
+def test_ssf_nonlinear_flow_conserves_its_own_energy(gpu):
+    """The conservative nonlinear equation (c5 = 1.3, the rest off) under
+    Strang splitting, dt = 1e-3, 200 steps on 128^2 with noise up to the
+    Nyquist frequency: the energy with the spectral kinetic term moves by the
+    splitting error only (2.8e-8 relative in the oracle's run of the same
+    case), the one with the order-5 stencil's kinetic term by 100 times more
+    (2.6e-6), since the scheme does not evolve the stencil's operator."""
+    n = 128
+    X, Y, h = configs.grid_2d(n, 0.2 * n)
+    r2 = X ** 2 + Y ** 2
+    rng = np.random.default_rng(3)
+    psi = (0.5 + 0.2j) * np.exp(-r2 / 6.0) + configs.noise(rng, r2.shape, 1e-2)
+    P = np.zeros((n, n))
+    c = np.zeros(23)
+    c[12], c[4] = 1.0, 1.3
+    out = np.asarray(native.solve_host(native.ProblemDesc.make(2, n, 5, 1e-3, h, c, bc="periodic", dtype="f64",
+                                                               scheme="ssf"), 200, P, psi, None)[0])
+    change = {}
+    for scheme in ("ssf", "rk4"):
+        d = native.ProblemDesc.make(2, n, 5, 1e-3, h, c, bc="periodic", dtype="f64", scheme=scheme)
+        e0 = native.energy(d, dev(P, "f64"), dev(psi, "f64"))["energy"]
+        e1 = native.energy(d, dev(P, "f64"), dev(out, "f64"))["energy"]
+        change[scheme] = abs(e1 - e0) / abs(e0)
+    assert change["ssf"] <= 1e-7, change
+    assert change["rk4"] >= 1e-6, change
