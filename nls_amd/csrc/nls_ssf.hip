@@ -261,6 +261,205 @@ __device__ __forceinline__ void fft_radix2_stage(LB buf, int logN, int logL, int
     __syncthreads();
 }
 
+// Stages whose inputs come from ld(l, e) and whose outputs go to st(l, e, v)
+// (element e of line l; q, r: the thread's butterfly and its output, known
+// at compile time once unrolled) instead of the LDS buffer: the first stage
+// of a transform can read its inputs from HBM, the last one write to HBM or
+// apply a pointwise factor on the way.  SYNC: synchronise between the reads and
+// the writes (outputs written in place over inputs other threads read); the
+// caller synchronises after the writes where they go to LDS.  TW: multiply by
+// the twiddles (the first stage's are all 1 and are skipped).  The butterfly
+// arithmetic is that of fft_radix8_stage / fft_radix4_stage.
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH, bool SYNC, bool TW, typename LD, typename ST>
+__device__ __forceinline__ void fft_radix8_io(int logN, int logL, int logNs, const cplx_t<T> *tw, cplx_t<T> w8,
+                                              LD ld, ST st) {
+    using C = cplx_t<T>;
+    constexpr int QB = (PTS / 8 + NTH - 1) / NTH;
+    const int lognb = logN - 3;
+    const int total = 1 << (logL + lognb);
+    const int Ns = 1 << logNs;
+    const int sh = logN - logNs - 3;
+    C v[QB][8], w[QB][7];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+            if (TW) {
+#pragma unroll
+                for (int r = 1; r < 8; ++r) w[q][r - 1] = tw[(r * k) << sh];
+            }
+#pragma unroll
+            for (int r = 0; r < 8; ++r) v[q][r] = ld(l, j + (r << lognb));
+        }
+    }
+    if (SYNC) __syncthreads();
+    if (INV) w8.y = -w8.y;
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+            if (TW) {
+#pragma unroll
+                for (int r = 1; r < 8; ++r) {
+                    C wr = w[q][r - 1];
+                    if (INV) wr.y = -wr.y;
+                    v[q][r] = cmul<T>(v[q][r], wr);
+                }
+            }
+            C e0 = v[q][0], e1 = v[q][2], e2 = v[q][4], e3 = v[q][6];
+            C o0 = v[q][1], o1 = v[q][3], o2 = v[q][5], o3 = v[q][7];
+            dft4<T, INV>(e0, e1, e2, e3);
+            dft4<T, INV>(o0, o1, o2, o3);
+            o1 = cmul<T>(o1, w8);
+            o2 = INV ? mk<T>(-o2.y, o2.x) : mk<T>(o2.y, -o2.x);
+            o3 = cmul<T>(o3, w8);
+            o3 = INV ? mk<T>(-o3.y, o3.x) : mk<T>(o3.y, -o3.x);
+            const int d = ((j >> logNs) << (logNs + 3)) + k;
+            st(l, d, mk<T>(e0.x + o0.x, e0.y + o0.y), q, 0);
+            st(l, d + Ns, mk<T>(e1.x + o1.x, e1.y + o1.y), q, 1);
+            st(l, d + 2 * Ns, mk<T>(e2.x + o2.x, e2.y + o2.y), q, 2);
+            st(l, d + 3 * Ns, mk<T>(e3.x + o3.x, e3.y + o3.y), q, 3);
+            st(l, d + 4 * Ns, mk<T>(e0.x - o0.x, e0.y - o0.y), q, 4);
+            st(l, d + 5 * Ns, mk<T>(e1.x - o1.x, e1.y - o1.y), q, 5);
+            st(l, d + 6 * Ns, mk<T>(e2.x - o2.x, e2.y - o2.y), q, 6);
+            st(l, d + 7 * Ns, mk<T>(e3.x - o3.x, e3.y - o3.y), q, 7);
+        }
+    }
+}
+
+template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH, bool SYNC, typename LD, typename ST>
+__device__ __forceinline__ void fft_radix4_io(int logN, int logL, int logNs, const cplx_t<T> *tw, LD ld, ST st) {
+    using C = cplx_t<T>;
+    constexpr int QB = (PTS / 4 + NTH - 1) / NTH;
+    const int lognb = logN - 2;
+    const int total = 1 << (logL + lognb);
+    const int Ns = 1 << logNs;
+    const int sh = logN - logNs - 2;
+    C v[QB][4], w[QB][3];
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < 4; ++r) w[q][r - 1] = tw[(r * k) << sh];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) v[q][r] = ld(l, j + (r << lognb));
+        }
+    }
+    if (SYNC) __syncthreads();
+#pragma unroll
+    for (int q = 0; q < QB; ++q) {
+        const int b = threadIdx.x + NTH * q;
+        if (b < total) {
+            const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> lognb);
+            const int j = LINE_FAST ? (b >> logL) : (b & ((1 << lognb) - 1));
+            const int k = j & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < 4; ++r) {
+                C wr = w[q][r - 1];
+                if (INV) wr.y = -wr.y;
+                v[q][r] = cmul<T>(v[q][r], wr);
+            }
+            const C t0 = mk<T>(v[q][0].x + v[q][2].x, v[q][0].y + v[q][2].y);
+            const C t1 = mk<T>(v[q][0].x - v[q][2].x, v[q][0].y - v[q][2].y);
+            const C t2 = mk<T>(v[q][1].x + v[q][3].x, v[q][1].y + v[q][3].y);
+            const C d13 = mk<T>(v[q][1].x - v[q][3].x, v[q][1].y - v[q][3].y);
+            const C t3 = INV ? mk<T>(-d13.y, d13.x) : mk<T>(d13.y, -d13.x);
+            const int d = ((j >> logNs) << (logNs + 2)) + k;
+            st(l, d, mk<T>(t0.x + t2.x, t0.y + t2.y), q, 0);
+            st(l, d + Ns, mk<T>(t1.x + t3.x, t1.y + t3.y), q, 1);
+            st(l, d + 2 * Ns, mk<T>(t0.x - t2.x, t0.y - t2.y), q, 2);
+            st(l, d + 3 * Ns, mk<T>(t1.x - t3.x, t1.y - t3.y), q, 3);
+        }
+    }
+}
+
+// The last stage of one 2048-point transform (radix 4, A) and the first of
+// the next (radix 8, B) without LDS in between.  A thread's two radix-4
+// butterflies of A produce the elements j0 + 256 s (s = 0 .. 7) of its line,
+// and those are exactly the inputs of its radix-8 butterfly of B; so the
+// outputs of A go through the pointwise map g(l, e, s, v) in registers (the
+// point's fields are the thread's slot s) straight into B, whose outputs go to
+// st(l, e, v, q, r).  One barrier between A's LDS reads and B's LDS writes
+// (B writes elements other threads read in A).  Arithmetic as in
+// fft_radix4_stage and fft_radix8_stage (B's twiddles are all 1, skipped).
+template <typename T, bool INV_A, bool INV_B, bool LINE_FAST, int PTS, int NTH, typename LD, typename G, typename ST>
+__device__ __forceinline__ void fft_2048_turn(int logL, const cplx_t<T> *tw, cplx_t<T> w8, LD ld, G g, ST st) {
+    using C = cplx_t<T>;
+    constexpr int LN = 11;
+    static_assert((PTS / 4 + NTH - 1) / NTH == 2 && (PTS / 8 + NTH - 1) / NTH == 1 && PTS == NTH * 8,
+                  "a thread holds two radix-4 and one radix-8 butterflies of its line");
+    C v[2][4], w[2][3];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const int b = threadIdx.x + NTH * q;
+        const int l = LINE_FAST ? (b & ((1 << logL) - 1)) : (b >> (LN - 2));
+        const int j = LINE_FAST ? (b >> logL) : (b & ((1 << (LN - 2)) - 1));
+#pragma unroll
+        for (int r = 1; r < 4; ++r) w[q][r - 1] = tw[r * j];       // k = j: the last stage, Ns = 512
+#pragma unroll
+        for (int r = 0; r < 4; ++r) v[q][r] = ld(l, j + (r << (LN - 2)));
+    }
+    __syncthreads();
+    const int bb = threadIdx.x;
+    const int l = LINE_FAST ? (bb & ((1 << logL) - 1)) : (bb >> (LN - 3));
+    const int j0 = LINE_FAST ? (bb >> logL) : (bb & ((1 << (LN - 3)) - 1));
+    C x[8];
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+#pragma unroll
+        for (int r = 1; r < 4; ++r) {
+            C wr = w[q][r - 1];
+            if (INV_A) wr.y = -wr.y;
+            v[q][r] = cmul<T>(v[q][r], wr);
+        }
+        const C t0 = mk<T>(v[q][0].x + v[q][2].x, v[q][0].y + v[q][2].y);
+        const C t1 = mk<T>(v[q][0].x - v[q][2].x, v[q][0].y - v[q][2].y);
+        const C t2 = mk<T>(v[q][1].x + v[q][3].x, v[q][1].y + v[q][3].y);
+        const C d13 = mk<T>(v[q][1].x - v[q][3].x, v[q][1].y - v[q][3].y);
+        const C t3 = INV_A ? mk<T>(-d13.y, d13.x) : mk<T>(d13.y, -d13.x);
+        // element j0 + 256 q + 512 r of the line: slot s = q + 2 r
+        x[q] = mk<T>(t0.x + t2.x, t0.y + t2.y);
+        x[q + 2] = mk<T>(t1.x + t3.x, t1.y + t3.y);
+        x[q + 4] = mk<T>(t0.x - t2.x, t0.y - t2.y);
+        x[q + 6] = mk<T>(t1.x - t3.x, t1.y - t3.y);
+    }
+    // the pointwise map one point at a time, after both butterflies (with
+    // the maps' temporaries interleaved, a SIMD would hold half the waves)
+#pragma unroll
+    for (int sl = 0; sl < 8; ++sl) {
+        x[sl] = g(l, j0 + 256 * sl, sl, x[sl]);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+    if (INV_B) w8.y = -w8.y;
+    C e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    C o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4<T, INV_B>(e0, e1, e2, e3);
+    dft4<T, INV_B>(o0, o1, o2, o3);
+    o1 = cmul<T>(o1, w8);
+    o2 = INV_B ? mk<T>(-o2.y, o2.x) : mk<T>(o2.y, -o2.x);
+    o3 = cmul<T>(o3, w8);
+    o3 = INV_B ? mk<T>(-o3.y, o3.x) : mk<T>(o3.y, -o3.x);
+    const int d = j0 << 3;                                          // Ns = 1
+    st(l, d, mk<T>(e0.x + o0.x, e0.y + o0.y), 0, 0);
+    st(l, d + 1, mk<T>(e1.x + o1.x, e1.y + o1.y), 0, 1);
+    st(l, d + 2, mk<T>(e2.x + o2.x, e2.y + o2.y), 0, 2);
+    st(l, d + 3, mk<T>(e3.x + o3.x, e3.y + o3.y), 0, 3);
+    st(l, d + 4, mk<T>(e0.x - o0.x, e0.y - o0.y), 0, 4);
+    st(l, d + 5, mk<T>(e1.x - o1.x, e1.y - o1.y), 0, 5);
+    st(l, d + 6, mk<T>(e2.x - o2.x, e2.y - o2.y), 0, 6);
+    st(l, d + 7, mk<T>(e3.x - o3.x, e3.y - o3.y), 0, 7);
+}
+
 // In-place FFT of 2^logL lines of length 2^logN (2^(logL+logN) <= PTS) in LDS.
 template <typename T, bool INV, bool LINE_FAST, int PTS, int NTH, typename LB>
 __device__ __forceinline__ void lds_fft(LB buf, int logN, int logL, int lstride, int estride,
@@ -337,8 +536,12 @@ __host__ __device__ constexpr int lines_per_wg(int logN, int logP) {
 }
 
 // Row pass: 2^(11-logN) rows per workgroup (at most n).
-template <typename S, typename T, bool DYN, int MODE>
-__global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<S, T> a) {
+// ROW2048: the middle row pass of a grid of 2048-point rows, a kernel of its
+// own, held to 128 registers (four waves per SIMD; in one kernel with the
+// general path, or left to the compiler with double-precision storage, it
+// took 131-136 and three).
+template <typename S, typename T, bool DYN, int MODE, bool ROW2048 = false>
+__global__ void __launch_bounds__(ROW_NTH, ROW2048 ? 4 : 1) k_ssf_rows(const SsfArgs<S, T> a) {
     using C = cplx_t<T>;
     constexpr int PPT = ROW_PTS / ROW_NTH;
     __shared__ C buf_mem[lds_padded(ROW_PTS, NLS_SSF_ROW_PAD)];
@@ -348,6 +551,53 @@ __global__ void __launch_bounds__(ROW_NTH) k_ssf_rows(const SsfArgs<S, T> a) {
     const int pts = 1 << (logL + logN);
     const size_t base = (size_t)blockIdx.x << (logL + logN);    // rows are contiguous
     T pp[PPT], nn[PPT];
+    if constexpr (ROW2048) {
+        static_assert(MODE == ROWS_MIDDLE && sizeof(T) == 8, "the 2048-point middle pass");
+        // one 2048-point row (8 x 8 x 8 x 4, compile-time sizes): the first
+        // inverse stage reads HBM; the last inverse stage, the two pointwise
+        // half steps and the first forward stage run in registers
+        // (fft_2048_turn: a thread's points are the elements tid + 256 s, its
+        // slots s of pp and nn); the last forward stage writes HBM -- six
+        // passes through LDS fewer and about half the barriers
+        constexpr int LN = ROW_LOG_PTS, NN = 1 << LN;
+        static_assert(PPT == 8, "a thread holds eight points of the row");
+        // the fields stay in the storage precision until used (exact
+        // conversion; single-precision storage then holds them in half the
+        // registers)
+        S ps[PPT], ns[PPT];
+#pragma unroll
+        for (int q = 0; q < PPT; ++q) {
+            const int idx = threadIdx.x + ROW_NTH * q;
+            ps[q] = a.P[base + idx];
+            ns[q] = DYN ? a.nres[base + idx] : (S)0;
+        }
+        auto lds_ld = [&](int, int e) { return buf[e]; };
+        auto lds_st = [&](int, int e, C v, int, int) { buf[e] = v; };
+        const C w8 = a.tw[NN >> 3];
+        fft_radix8_io<T, true, false, ROW_PTS, ROW_NTH, false, false>(
+            LN, 0, 0, a.tw, w8, [&](int, int e) { return widen<T, S>(a.psi[base + e]); }, lds_st);
+        __syncthreads();
+        fft_radix8_stage<T, true, false, ROW_PTS, ROW_NTH>(buf, LN, 0, NN, 1, 3, a.tw, w8);
+        fft_radix8_stage<T, true, false, ROW_PTS, ROW_NTH>(buf, LN, 0, NN, 1, 6, a.tw, w8);
+        fft_2048_turn<T, true, false, false, ROW_PTS, ROW_NTH>(
+            0, a.tw, w8, lds_ld,
+            [&](int, int e, int slot, C v) {
+                T nv = (T)ns[slot];
+                const T pv = (T)ps[slot];
+                pointwise_rk4<T, DYN>(a.c, a.inv_c11, pv, a.half_dt, v, nv);
+                pointwise_rk4<T, DYN>(a.c, a.inv_c11, pv, a.half_dt, v, nv);
+                // the reservoir is final here (the transforms act on psi only)
+                if (DYN) a.nres[base + e] = (S)nv;
+                return v;
+            },
+            lds_st);
+        __syncthreads();
+        fft_radix8_stage<T, false, false, ROW_PTS, ROW_NTH>(buf, LN, 0, NN, 1, 3, a.tw, w8);
+        fft_radix8_stage<T, false, false, ROW_PTS, ROW_NTH>(buf, LN, 0, NN, 1, 6, a.tw, w8);
+        fft_radix4_io<T, false, false, ROW_PTS, ROW_NTH, false>(
+            LN, 0, 9, a.tw, lds_ld, [&](int, int e, C v, int, int) { a.psi[base + e] = narrow<S, T>(v); });
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int idx = threadIdx.x + ROW_NTH * q;
@@ -411,6 +661,34 @@ __global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<S, 
     const int logL = ColShape<T>::LOG_L < logN ? ColShape<T>::LOG_L : logN;
     const int L = 1 << logL, pts = L << logN;
     const int col0 = blk << logL;
+    constexpr int LL = ColShape<T>::LOG_L;
+    if (sizeof(T) == 8 && logN == ROW_LOG_PTS) {
+        // 2048-point columns (8 x 8 x 8 x 4, compile-time sizes): the first
+        // forward stage reads HBM, the last forward stage, the phase and the
+        // first inverse stage run in registers (fft_2048_turn), and the last
+        // inverse stage writes HBM -- six passes through LDS (reads and
+        // writes) fewer and half the barriers of load / transform / phase /
+        // transform / store
+        constexpr int LN = ROW_LOG_PTS, NN = 1 << LN, LW = 1 << LL;
+        auto lds_ld = [&](int l, int e) { return buf[l + e * LW]; };
+        auto lds_st = [&](int l, int e, C v, int, int) { buf[l + e * LW] = v; };
+        const C w8 = a.tw[NN >> 3];
+        fft_radix8_io<T, false, true, COL_PTS, COL_NTH, false, false>(
+            LN, LL, 0, a.tw, w8, [&](int l, int e) { return widen<T, S>(a.psi[(size_t)e * NN + col0 + l]); }, lds_st);
+        __syncthreads();
+        fft_radix8_stage<T, false, true, COL_PTS, COL_NTH>(buf, LN, LL, 1, LW, 3, a.tw, w8);
+        fft_radix8_stage<T, false, true, COL_PTS, COL_NTH>(buf, LN, LL, 1, LW, 6, a.tw, w8);
+        fft_2048_turn<T, false, true, true, COL_PTS, COL_NTH>(
+            LL, a.tw, w8, lds_ld, [&](int l, int e, int, C v) { return cmul<T>(v, cmul<T>(a.ph[e], a.ph[col0 + l])); },
+            lds_st);
+        __syncthreads();
+        fft_radix8_stage<T, true, true, COL_PTS, COL_NTH>(buf, LN, LL, 1, LW, 3, a.tw, w8);
+        fft_radix8_stage<T, true, true, COL_PTS, COL_NTH>(buf, LN, LL, 1, LW, 6, a.tw, w8);
+        fft_radix4_io<T, true, true, COL_PTS, COL_NTH, false>(
+            LN, LL, 9, a.tw, lds_ld,
+            [&](int l, int e, C v, int, int) { a.psi[(size_t)e * NN + col0 + l] = narrow<S, T>(v); });
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < PPT; ++q) {
         const int idx = threadIdx.x + COL_NTH * q;
@@ -421,7 +699,6 @@ __global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<S, 
     }
     __syncthreads();
     // 2048-point columns: the transform with compile-time sizes, as in the row pass
-    constexpr int LL = ColShape<T>::LOG_L;
     if (logN == ROW_LOG_PTS) lds_fft<T, false, true, COL_PTS, COL_NTH>(buf, ROW_LOG_PTS, LL, 1, 1 << LL, a.tw);
     else lds_fft<T, false, true, COL_PTS, COL_NTH>(buf, logN, logL, 1, L, a.tw);
 #pragma unroll
@@ -589,7 +866,12 @@ static int ssf_run(const nls_problem *p, int iters, const S *P, cplx_t<S> *psi, 
     for (int it = 0; it < iters; ++it) {
         hipLaunchKernelGGL((k_ssf_cols<S, T>), dim3(8 * chunk), dim3(ColShape<T>::NTH), 0, s, a, cblocks, chunk);
         if (it + 1 < iters)
-            hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_MIDDLE>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
+        {
+            if (logN == ROW_LOG_PTS)
+                hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_MIDDLE, true>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
+            else
+                hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_MIDDLE>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
+        }
         else
             hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_LAST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
     }
