@@ -791,6 +791,110 @@ __global__ void __launch_bounds__(LineShape<T>::NTH) k_ssf_line(const SsfArgs<S,
     }
 }
 
+// Radix-8 butterfly of Stockham stage logNs of a 2^LN-point transform on
+// the inputs x[0..7] (elements j + r 2^(LN-3) of the line), in place: x[r]
+// becomes the output element d + r Ns, d = ((j >> logNs) << (logNs + 3)) +
+// (j & (Ns - 1)).  The arithmetic of fft_radix8_stage; TW = false skips the
+// twiddles of the first stage, which are all 1.
+template <typename T, bool INV, bool TW>
+__device__ __forceinline__ void bfly8(cplx_t<T> x[8], int j, int logNs, int LN, const cplx_t<T> *tw, cplx_t<T> w8) {
+    using C = cplx_t<T>;
+    if (TW) {
+        const int k = j & ((1 << logNs) - 1), sh = LN - logNs - 3;
+#pragma unroll
+        for (int r = 1; r < 8; ++r) {
+            C wr = tw[(r * k) << sh];
+            if (INV) wr.y = -wr.y;
+            x[r] = cmul<T>(x[r], wr);
+        }
+    }
+    if (INV) w8.y = -w8.y;
+    C e0 = x[0], e1 = x[2], e2 = x[4], e3 = x[6];
+    C o0 = x[1], o1 = x[3], o2 = x[5], o3 = x[7];
+    dft4<T, INV>(e0, e1, e2, e3);
+    dft4<T, INV>(o0, o1, o2, o3);
+    o1 = cmul<T>(o1, w8);
+    o2 = INV ? mk<T>(-o2.y, o2.x) : mk<T>(o2.y, -o2.x);
+    o3 = cmul<T>(o3, w8);
+    o3 = INV ? mk<T>(-o3.y, o3.x) : mk<T>(o3.y, -o3.x);
+    x[0] = mk<T>(e0.x + o0.x, e0.y + o0.y);
+    x[1] = mk<T>(e1.x + o1.x, e1.y + o1.y);
+    x[2] = mk<T>(e2.x + o2.x, e2.y + o2.y);
+    x[3] = mk<T>(e3.x + o3.x, e3.y + o3.y);
+    x[4] = mk<T>(e0.x - o0.x, e0.y - o0.y);
+    x[5] = mk<T>(e1.x - o1.x, e1.y - o1.y);
+    x[6] = mk<T>(e2.x - o2.x, e2.y - o2.y);
+    x[7] = mk<T>(e3.x - o3.x, e3.y - o3.y);
+}
+
+// 1-D, n = 4096 = 8^4 (the longest line): the line's values stay in
+// registers.  Every radix-8 stage of this transform reads the elements
+// j + 512 r of thread j's butterfly, and the last stage writes them back to
+// the same elements, so a thread's eight points (j + 512 s) are its inputs of
+// the first stage and its outputs of the last: the pointwise half steps and
+// the phase run in registers, and only the three stage-to-stage exchanges of
+// a transform go through LDS, alternating between two buffers (one barrier
+// each instead of two).  Six barriers a step instead of about nineteen.
+template <typename S, typename T, bool DYN>
+__global__ void __launch_bounds__(512) k_ssf_line4096(const SsfArgs<S, T> a) {
+    using C = cplx_t<T>;
+    constexpr int LN = 12, NN = 1 << LN, NTH = NN / 8;
+    static_assert(sizeof(T) == 8, "double-precision transforms");
+    __shared__ C mem[2][lds_padded(NN, NLS_SSF_LINE_PAD)];
+    const Lds<T, NLS_SSF_LINE_PAD> A{mem[0]}, B{mem[1]};
+    const int j = threadIdx.x;
+    C x[8], ph[8];
+    T pp[8], nn[8];
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int e = j + NTH * r;
+        x[r] = widen<T, S>(a.psi[e]);
+        ph[r] = a.ph[e];
+        pp[r] = (T)a.P[e];
+        nn[r] = DYN ? (T)a.nres[e] : (T)0;
+    }
+    const C w8 = a.tw[NN >> 3];
+    // outputs of stage logNs to buffer L (elements d + r Ns), then the
+    // thread's inputs of the next stage (elements j + 512 r)
+    auto exchange = [&](const Lds<T, NLS_SSF_LINE_PAD> &L, int logNs) {
+        const int Ns = 1 << logNs;
+        const int d = ((j >> logNs) << (logNs + 3)) + (j & (Ns - 1));
+#pragma unroll
+        for (int r = 0; r < 8; ++r) L[d + r * Ns] = x[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < 8; ++r) x[r] = L[j + NTH * r];
+    };
+    for (int it = 0; it < a.iters; ++it) {
+#pragma unroll
+        for (int r = 0; r < 8; ++r) pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[r], a.half_dt, x[r], nn[r]);
+        bfly8<T, false, false>(x, j, 0, LN, a.tw, w8);
+        exchange(A, 0);
+        bfly8<T, false, true>(x, j, 3, LN, a.tw, w8);
+        exchange(B, 3);
+        bfly8<T, false, true>(x, j, 6, LN, a.tw, w8);
+        exchange(A, 6);
+        bfly8<T, false, true>(x, j, 9, LN, a.tw, w8);     // outputs j + 512 r: the thread's own points
+#pragma unroll
+        for (int r = 0; r < 8; ++r) x[r] = cmul<T>(x[r], ph[r]);
+        bfly8<T, true, false>(x, j, 0, LN, a.tw, w8);
+        exchange(B, 0);
+        bfly8<T, true, true>(x, j, 3, LN, a.tw, w8);
+        exchange(A, 3);
+        bfly8<T, true, true>(x, j, 6, LN, a.tw, w8);
+        exchange(B, 6);
+        bfly8<T, true, true>(x, j, 9, LN, a.tw, w8);
+#pragma unroll
+        for (int r = 0; r < 8; ++r) pointwise_rk4<T, DYN>(a.c, a.inv_c11, pp[r], a.half_dt, x[r], nn[r]);
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+        const int e = j + NTH * r;
+        a.psi[e] = narrow<S, T>(x[r]);
+        if (DYN) a.nres[e] = (S)nn[r];
+    }
+}
+
 // The unit complex number (c, s) in T.  In single precision the rounded
 // pair has |.|^2 = 1 + O(6e-8), a fixed gain or loss of the mode it
 // multiplies on every use; of the pairs within one ulp of (c, s) the one
@@ -856,7 +960,8 @@ static int ssf_run(const nls_problem *p, int iters, const S *P, cplx_t<S> *psi, 
     a.psi = psi; a.nres = nres; a.P = P; a.tw = tw; a.ph = ph;
     if (p->dim == 1) {
         a.iters = iters;
-        hipLaunchKernelGGL((k_ssf_line<S, T, DYN>), dim3(1), dim3(LineShape<T>::NTH), 0, s, a);
+        if (logN == 12) hipLaunchKernelGGL((k_ssf_line4096<S, T, DYN>), dim3(1), dim3(512), 0, s, a);
+        else hipLaunchKernelGGL((k_ssf_line<S, T, DYN>), dim3(1), dim3(LineShape<T>::NTH), 0, s, a);
         return hip_check(hipGetLastError(), "ssf line launch");
     }
     const int rblocks = N >> lines_per_wg(logN, ROW_LOG_PTS);

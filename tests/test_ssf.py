@@ -146,14 +146,20 @@ def test_gpu_ssf_f32_within_tolerance(gpu, dim, n):
 
 
 @pytest.mark.gpu
-def test_gpu_ssf_2048_f64(gpu):
+@pytest.mark.parametrize("dtype,dyn", [("f64", True), ("f64", False), ("f32", False)])
+def test_gpu_ssf_2048(gpu, dtype, dyn):
+    """2048-point lines take their own row and column kernels (first and last
+    FFT stages against HBM, the turn between transforms in registers): every
+    storage type and reservoir mode against the oracle."""
     from nls_amd import native
-    h, P, psi0, n0, c = ssf_case(2, 2048, True)
-    desc = native.ProblemDesc.make(2, 2048, 5, 2e-3, h, c, bc="periodic", dtype="f64", scheme="ssf")
+    h, P, psi0, n0, c = ssf_case(2, 2048, dyn)
+    desc = native.ProblemDesc.make(2, 2048, 5, 2e-3, h, c, bc="periodic", dtype=dtype, scheme="ssf")
     psi, nres = native.solve_host(desc, 3, P, psi0, n0)
     opsi, onres = S.ssf(2, 2048, h, 2e-3, 3, P, c, psi0, n0)
-    assert rel(psi, opsi) <= TOL64
-    assert rel(nres, onres) <= TOL64
+    tol = TOL64 if dtype == "f64" else TOL32
+    assert rel(psi, opsi) <= tol
+    if dyn:
+        assert rel(nres, onres) <= tol
 
 
 def test_ssf_argument_errors():
