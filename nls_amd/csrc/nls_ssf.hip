@@ -50,10 +50,14 @@ template <typename T> struct LineShape { static constexpr int NTH = sizeof(T) ==
 // Column pass: 2^LOG_L adjacent columns per workgroup (fewer when n is
 // smaller), so each row segment it reads and writes is 32 bytes: two FP64
 // or four FP32 columns.  More columns per workgroup would only leave fewer
-// workgroups for small grids.
-template <typename T> struct ColShape;
-template <> struct ColShape<double> { static constexpr int LOG_L = 1, PTS = 4096, NTH = 512; };
-template <> struct ColShape<float> { static constexpr int LOG_L = 2, PTS = 8192, NTH = 1024; };
+// workgroups for small grids.  WIDE: the 2048-point column pass (stages in
+// registers), four columns per workgroup of 1024 threads, 136 KB of LDS, one
+// workgroup per CU: 64-byte row segments (2048^2 per step, same box, two
+// rounds: FP64 99.6 -> 96.6 us, FP32 storage 89.9 -> 88.1 us; one column
+// per workgroup, 16-byte segments: 123 / 108 us).
+template <typename T, bool WIDE = false> struct ColShape;
+template <> struct ColShape<double, false> { static constexpr int LOG_L = 1, PTS = 4096, NTH = 512; };
+template <> struct ColShape<double, true> { static constexpr int LOG_L = 2, PTS = 8192, NTH = 1024; };
 
 // LDS layout of the transforms: element i at i + (i >> SHIFT), one complex of
 // padding per 2^SHIFT, so that the strided writes of the first radix-8
@@ -646,23 +650,26 @@ __global__ void __launch_bounds__(ROW_NTH, ROW2048 ? 4 : 1) k_ssf_rows(const Ssf
 // Column pass: min(2^LOG_L, n) adjacent columns per workgroup, LDS
 // [row][col].  Workgroups of one XCD take adjacent column blocks
 // (round-robin dispatch).
-template <typename S, typename T>
-__global__ void __launch_bounds__(ColShape<T>::NTH) k_ssf_cols(const SsfArgs<S, T> a, int nblocks, int chunk) {
+template <typename S, typename T, bool COL2048 = false>
+__global__ void __launch_bounds__((ColShape<T, COL2048>::NTH)) k_ssf_cols(const SsfArgs<S, T> a, int nblocks,
+                                                                           int chunk) {
     using C = cplx_t<T>;
-    constexpr int COL_PTS = ColShape<T>::PTS, COL_NTH = ColShape<T>::NTH;
+    using CS = ColShape<T, COL2048>;
+    constexpr int COL_PTS = CS::PTS, COL_NTH = CS::NTH;
     constexpr int PPT = COL_PTS / COL_NTH;
-    static_assert(COL_PTS == 2048 << ColShape<T>::LOG_L, "column block of 2048-point columns");
+    static_assert(COL_PTS == 2048 << CS::LOG_L, "column block of 2048-point columns");
     __shared__ C buf_mem[lds_padded(COL_PTS, NLS_SSF_COL_PAD)];
     const Lds<T, NLS_SSF_COL_PAD> buf{buf_mem};
     const int b = blockIdx.x;
     const int blk = (b & 7) * chunk + (b >> 3);
     if (blk >= nblocks) return;
     const int logN = a.logN, N = 1 << logN;
-    const int logL = ColShape<T>::LOG_L < logN ? ColShape<T>::LOG_L : logN;
+    const int logL = CS::LOG_L < logN ? CS::LOG_L : logN;
     const int L = 1 << logL, pts = L << logN;
     const int col0 = blk << logL;
-    constexpr int LL = ColShape<T>::LOG_L;
-    if (sizeof(T) == 8 && logN == ROW_LOG_PTS) {
+    constexpr int LL = CS::LOG_L;
+    if constexpr (COL2048) {
+        static_assert(sizeof(T) == 8, "the 2048-point column pass");
         // 2048-point columns (8 x 8 x 8 x 4, compile-time sizes): the first
         // forward stage reads HBM, the last forward stage, the phase and the
         // first inverse stage run in registers (fft_2048_turn), and the last
@@ -965,11 +972,17 @@ static int ssf_run(const nls_problem *p, int iters, const S *P, cplx_t<S> *psi, 
         return hip_check(hipGetLastError(), "ssf line launch");
     }
     const int rblocks = N >> lines_per_wg(logN, ROW_LOG_PTS);
-    const int cblocks = N >> (ColShape<T>::LOG_L < logN ? ColShape<T>::LOG_L : logN);
+    const bool wide = logN == ROW_LOG_PTS;
+    const int clog = wide ? ColShape<T, true>::LOG_L : ColShape<T>::LOG_L;
+    const int cblocks = N >> (clog < logN ? clog : logN);
     const int chunk = (cblocks + 7) / 8;
     hipLaunchKernelGGL((k_ssf_rows<S, T, DYN, ROWS_FIRST>), dim3(rblocks), dim3(ROW_NTH), 0, s, a);
     for (int it = 0; it < iters; ++it) {
-        hipLaunchKernelGGL((k_ssf_cols<S, T>), dim3(8 * chunk), dim3(ColShape<T>::NTH), 0, s, a, cblocks, chunk);
+        if (wide)
+            hipLaunchKernelGGL((k_ssf_cols<S, T, true>), dim3(8 * chunk), dim3(ColShape<T, true>::NTH), 0, s, a,
+                               cblocks, chunk);
+        else
+            hipLaunchKernelGGL((k_ssf_cols<S, T>), dim3(8 * chunk), dim3(ColShape<T>::NTH), 0, s, a, cblocks, chunk);
         if (it + 1 < iters)
         {
             if (logN == ROW_LOG_PTS)
