@@ -151,10 +151,11 @@ def cpu_baseline(members, steps):
 def single_grid_2048(dev, steps=100):
     """One 2048 x 2048 periodic grid (h = 0.0625, order 5) on the plain path:
     FP64 and FP32, dynamic reservoir, each RK4 engine and the split-step
-    Fourier scheme.  Rates in grid-point-steps/s; the HBM fraction of each is
-    given on two bases: the per-stage (k-storage) scheme's 15 S + 4 r bytes
-    per point-step, whose 8 TB/s ceiling is the reference roofline of this
-    path, and the compulsory 2 S + r (SSF: 4 c + 3 r)."""
+    Fourier scheme, each the median of three timed runs of `steps` steps.
+    Rates in grid-point-steps/s; the HBM fraction of each is given on two
+    bases: the per-stage (k-storage) scheme's 15 S + 4 r bytes per
+    point-step, whose 8 TB/s ceiling is the reference roofline of this path,
+    and the compulsory 2 S + r (SSF: 4 c + 3 r)."""
     import torch
     from nls_amd import configs, native
     from nls_amd.autotune import Tuner
@@ -180,10 +181,15 @@ def single_grid_2048(dev, steps=100):
             ws = torch.empty(max(1, native.workspace_bytes(d)), dtype=torch.uint8, device=dev)
             native.advance(d, 4, P, psi, nres, ws)
             torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            native.advance(d, steps, P, psi, nres, ws)
-            torch.cuda.synchronize()
-            t = time.perf_counter() - t0
+            # median of three timed runs: the first run after the grid's
+            # allocation can be several percent slow (clock ramp)
+            ts = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                native.advance(d, steps, P, psi, nres, ws)
+                torch.cuda.synchronize()
+                ts.append(time.perf_counter() - t0)
+            t = sorted(ts)[1]
             rate = n * n * steps / t
             r = {"grid_point_steps_per_s": rate, "us_per_step": t / steps * 1e6,
                  "hbm_frac_compulsory_bytes": rate * mbytes / HBM_PEAK}
