@@ -53,7 +53,8 @@ def solve(dim, n, dtype, engine, scheme, steps, static, seed):
 CASES = [(2, 512, 200, "f32", "fused", "rk4"), (2, 256, 96, "f64", "fused", "rk4"),
          (2, 300, 130, "f64", "stage", "rk4"), (1, 6000, 5000, "f32", "stage", "rk4"),
          (1, 4096, 700, "f64", "resident", "rk4"), (2, 64, 40, "f32", "resident", "rk4"),
-         (2, 512, 128, "f32", "auto", "ssf"), (1, 2048, 256, "f64", "auto", "ssf")]
+         (2, 512, 128, "f32", "auto", "ssf"), (1, 2048, 256, "f64", "auto", "ssf"),
+         (2, 2048, 1024, "f32", "auto", "ssf"), (1, 4096, 1024, "f64", "auto", "ssf")]
 
 
 @pytest.mark.parametrize("dim,n1,n2,dtype,engine,scheme", CASES)
