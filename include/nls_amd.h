@@ -275,8 +275,9 @@ int nls_observables(const nls_problem *p, const void *psi, const void *nres, dou
  * reservoir, else c12 P / (c13 + c14 |psi|^2)), out[0] = their sum; dV as
  * nls_observables (2*pi*r*dr with the weight r on the radial grid).  The
  * reference defines no energy; its chemical potential is the same operator's
- * expectation, Re mu * norm = out[1] + 2 out[2] + out[3] (mu with the
- * finite-difference operator).  L is the order-p finite-difference stencil
+ * expectation, Re mu * norm = out[1] + 2 out[2] + out[3] for RK4 problems
+ * (nls_chemical_potential takes the finite-difference operator for every
+ * scheme).  L is the order-p finite-difference stencil
  * for RK4 problems and the spectral operator for split-step (SSF) problems:
  * out[1] = (dV / N^d) sum_k |k|^2 |psi_hat(k)|^2 over the grid's
  * wavenumbers k = 2 pi f / (n dx), f = -n/2 .. n/2 - 1 (taken from the 1-D
