@@ -171,9 +171,13 @@ def single_grid_2048(dev, steps=100):
         rt = torch.float64 if dtype == "f64" else torch.float32
         ct = torch.complex128 if dtype == "f64" else torch.complex64
         P = torch.ones((n, n), dtype=rt, device=dev)
-        for name, kw, kbytes, mbytes in (("rk4_fused", dict(engine="fused"), 15 * S + 4 * rs, 2 * S + rs),
-                                          ("rk4_stage", dict(engine="stage"), 15 * S + 4 * rs, 2 * S + rs),
-                                          ("ssf", dict(scheme="ssf"), None, 4 * cs + 3 * rs)):
+        legs = [("rk4_fused", dict(engine="fused"), 15 * S + 4 * rs, 2 * S + rs),
+                ("rk4_stage", dict(engine="stage"), 15 * S + 4 * rs, 2 * S + rs),
+                ("ssf", dict(scheme="ssf"), None, 4 * cs + 3 * rs)]
+        if dtype == "f32":
+            # double-precision arithmetic on the single-precision fields
+            legs.append(("rk4_fused_f64arith", dict(engine="fused", f64_arith=True), None, 2 * S + rs))
+        for name, kw, kbytes, mbytes in legs:
             d = native.ProblemDesc.make(2, n, 5, 1e-4, 0.0625, configs.model_coeffs(dynamic=True), bc="periodic",
                                         dtype=dtype, **kw)
             psi = torch.full((n, n), 0.1, dtype=ct, device=dev)
@@ -293,9 +297,14 @@ def config_accuracy(dev):
              ("square2048_rk4", configs.square2048(), "rk4"), ("square2048_ssf", configs.square2048(), "ssf")]
     for name, inst, scheme in cases:
         res, obs = {}, {}
-        for dtype in ("f64", "f32"):
+        # square2048 RK4 also with double-precision arithmetic on the
+        # single-precision fields (NLS_FLAG_F64_ARITH)
+        variants = [("f64", "f64", False), ("f32", "f32", False)]
+        if name == "square2048_rk4":
+            variants.append(("f32_f64arith", "f32", True))
+        for key, dtype, wide in variants:
             d = native.ProblemDesc.make(inst.dim, inst.n, 5, inst.dt, inst.dx, inst.coeffs, bc="periodic",
-                                        dtype=dtype, scheme=scheme)
+                                        dtype=dtype, scheme=scheme, f64_arith=wide)
             rt = torch.float64 if dtype == "f64" else torch.float32
             ct = torch.complex128 if dtype == "f64" else torch.complex64
             P = torch.as_tensor(np.ascontiguousarray(inst.P), dtype=rt, device=dev)
@@ -304,13 +313,18 @@ def config_accuracy(dev):
             ws = torch.empty(max(1, native.workspace_bytes(d)), dtype=torch.uint8, device=dev)
             native.advance(d, inst.steps, P, psi, nres, ws)
             torch.cuda.synchronize()
-            res[dtype] = psi.cpu().numpy().astype(np.complex128)
-            obs[dtype] = {"norm": native.observables(d, psi, nres)["norm"],
-                          "energy": native.energy(d, P, psi, nres)["energy"]}
+            res[key] = psi.cpu().numpy().astype(np.complex128)
+            obs[key] = {"norm": native.observables(d, psi, nres)["norm"],
+                        "energy": native.energy(d, P, psi, nres)["energy"]}
         err = float(np.linalg.norm(res["f32"] - res["f64"]) / np.linalg.norm(res["f64"]))
         out[name] = {"relL2_f32_vs_f64": err, "steps": inst.steps, "points": inst.points,
                      "fp64_accumulated": {"norm_f64": obs["f64"]["norm"], "norm_f32": obs["f32"]["norm"],
                                           "energy_f64": obs["f64"]["energy"], "energy_f32": obs["f32"]["energy"]}}
+        if "f32_f64arith" in res:
+            out[name]["relL2_f32_f64arith_vs_f64"] = float(np.linalg.norm(res["f32_f64arith"] - res["f64"]) /
+                                                         np.linalg.norm(res["f64"]))
+            log("accuracy %s: FP32 fields with FP64 arithmetic vs FP64 relL2 = %.2e" % (
+                name, out[name]["relL2_f32_f64arith_vs_f64"]))
         # nls_energy's kinetic term uses the Laplacian the scheme evolves
         out[name]["fp64_accumulated"]["energy_kinetic_operator"] = (
             "spectral" if scheme == "ssf" else "finite-difference order %d" % inst.order)

@@ -50,6 +50,11 @@ extern "C" {
 /* nls_problem.flags */
 #define NLS_FLAG_EAGER    1   /* issue every launch directly: no step-block
                                  graphs (nls_advance)                        */
+#define NLS_FLAG_F64_ARITH 2  /* single-precision fields (dtype NLS_F32),
+                                 double-precision RK4 arithmetic: the fused
+                                 engine's double-precision step on complex64 /
+                                 float arrays, rounded once per step on store
+                                 (2-D periodic RK4 only; not in batches)     */
 /* RK4 engines (nls_problem.engine, nls_batch_create) */
 #define NLS_ENGINE_AUTO  -1   /* RESIDENT if the instance (every member of a
                                  batch) fits, else FUSED for 2-D and STAGE

@@ -24,6 +24,7 @@
 //                       (2-D) or per-stage (1-D) engine.
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 #include <cstring>
 #include <vector>
 
@@ -265,9 +266,15 @@ static_assert(RingMap<80, 48, 2, 84>().v[2048] == (6 * 84 + 6 | 6 << 16 | 6 << 2
 static_assert(RingMap<80, 32, 2, 84>().v[1024] == (6 * 84 + 6 | 6 << 16 | 6 << 24), "stage-3 ring follows the 64 x 16 tile");
 template <int W, int HH, int R, int WS> __device__ const RingMap<W, HH, R, WS> ring_map{};
 
-template <typename T, int ORDER, int BC, bool DYN, int FORM>
+// S32 (double precision only): the fields sit in HBM in single precision --
+// state, reservoir and pumping are widened on load and the new state rounded
+// once on store -- and everything else is the double-precision step: FP64
+// stage arithmetic and LDS at FP32 storage bytes.  The FusedArgs pointers
+// then hold the single-precision arrays.
+template <typename T, int ORDER, int BC, bool DYN, int FORM, bool S32 = false>
 __global__ void __launch_bounds__((FusedTile<T, ORDER, FORM>::NT), (FusedTile<T, ORDER, FORM>::WAVES))
 k_fused_step_2d(const FusedArgs<T> f) {
+    static_assert(!S32 || sizeof(T) == 8, "single-precision storage under double-precision arithmetic");
     using C = cplx_t<T>;
     using FT = FusedTile<T, ORDER, FORM>;
     constexpr int R = (ORDER - 1) / 2, H = 4 * R;
@@ -301,11 +308,13 @@ k_fused_step_2d(const FusedArgs<T> f) {
     const int n = m.n;
     const int ylim = BC == BC_SLAB ? f.slab_end : n;   // output rows end here
     const long long off = m.off;
-    const C *u = f.u + off;
-    const T *un = DYN ? f.un + off : nullptr;
-    const T *P = f.P + off;
-    C *uo = f.uo + off;
-    T *uno = DYN ? f.uno + off : nullptr;
+    using SC = std::conditional_t<S32, float2, C>;      // storage types
+    using SR = std::conditional_t<S32, float, T>;
+    const SC *u = (const SC *)f.u + off;
+    const SR *un = DYN ? (const SR *)f.un + off : nullptr;
+    const SR *P = (const SR *)f.P + off;
+    SC *uo = (SC *)f.uo + off;
+    SR *uno = DYN ? (SR *)f.uno + off : nullptr;
     const int tid = threadIdx.x;
     // single precision: the region loads take 32-bit byte offsets from
     // buffer resources over the member's arrays (fused_holds: under 4 GiB)
@@ -416,9 +425,10 @@ k_fused_step_2d(const FusedArgs<T> f) {
                 } else {
                     // double precision keeps 64-bit addresses: its 512-thread
                     // form has no registers to spare for the buffer resources
-                    uu[j] = u[gi];
-                    cp[j] = P[gi];
-                    if (DYN) nn[j] = un[gi];
+                    const SC v = u[gi];
+                    uu[j] = mk<T>((T)v.x, (T)v.y);
+                    cp[j] = (T)P[gi];
+                    if (DYN) nn[j] = (T)un[gi];
                 }
             }
             if (!GROUPED) ya[li[j]] = uu[j];
@@ -537,8 +547,9 @@ k_fused_step_2d(const FusedArgs<T> f) {
         T kn;
         stage_rhs(yb, j, k, kn);
         const size_t gi = (size_t)gy * n + gx;
-        uo[gi] = mk<T>(uu[j].x + (acc[j].x + k.x) * dt6, uu[j].y + (acc[j].y + k.y) * dt6);
-        if (DYN) uno[gi] = nn[j] + (accn[j] + kn) * dt6;
+        const C v = mk<T>(uu[j].x + (acc[j].x + k.x) * dt6, uu[j].y + (acc[j].y + k.y) * dt6);
+        uo[gi] = SC{(SR)v.x, (SR)v.y};
+        if (DYN) uno[gi] = (SR)(nn[j] + (accn[j] + kn) * dt6);
     }
 }
 
@@ -1020,7 +1031,7 @@ static void launch_batch_stage(int dim, int order, int bc, bool dyn, const Batch
 // build (order 7 has one form; FORM_HALF is double precision without
 // reflecting boundaries; FORM_TALL512 is the single-precision pair kernel)
 // fall back to FORM_TALL, which fused_form never asks otherwise.
-template <typename T, int O, int B, bool D>
+template <typename T, int O, int B, bool D, bool S32 = false>
 static void launch_fused_form(int form, const FusedArgs<T> &f, hipStream_t s) {
     const dim3 grid(8 * f.chunk);
     // periodic grids (the variants with reflecting boundaries or row slabs
@@ -1036,18 +1047,28 @@ static void launch_fused_form(int form, const FusedArgs<T> &f, hipStream_t s) {
     }
     if constexpr (O != 7) {
         if (form == FORM_SHORT) {
-            hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_SHORT>), grid, dim3(FusedTile<T, O, FORM_SHORT>::NT),
-                               0, s, f);
+            hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_SHORT, S32>), grid,
+                               dim3(FusedTile<T, O, FORM_SHORT>::NT), 0, s, f);
             return;
         }
         if constexpr (sizeof(T) == 8 && B != BC_REF)
             if (form == FORM_HALF) {
-                hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_HALF>), grid,
+                hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_HALF, S32>), grid,
                                    dim3(FusedTile<T, O, FORM_HALF>::NT), 0, s, f);
                 return;
             }
     }
-    hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_TALL>), grid, dim3(FusedTile<T, O, FORM_TALL>::NT), 0, s, f);
+    hipLaunchKernelGGL((k_fused_step_2d<T, O, B, D, FORM_TALL, S32>), grid, dim3(FusedTile<T, O, FORM_TALL>::NT), 0,
+                       s, f);
+}
+
+// Double-precision arithmetic on single-precision fields (NLS_FLAG_F64_ARITH):
+// periodic grids, the double-precision forms.
+static void launch_fused_s32(int order, bool dyn, int form, const FusedArgs<double> &f, hipStream_t s) {
+#define NLS_CASE(O, D) \
+    if (order == O && dyn == D) return launch_fused_form<double, O, BC_PERIODIC, D, true>(form, f, s);
+    NLS_CASE(3, false) NLS_CASE(3, true) NLS_CASE(5, false) NLS_CASE(5, true) NLS_CASE(7, false) NLS_CASE(7, true)
+#undef NLS_CASE
 }
 
 template <typename T>
@@ -1300,8 +1321,49 @@ int resident_advance(const nls_problem *p, int iters, const void *pumping, void 
     return resident_run<float>(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);
 }
 
+// NLS_FLAG_F64_ARITH: the double-precision step (forms, member descriptor,
+// arithmetic) on single-precision arrays.  The FusedArgs pointers carry the
+// float arrays; the kernel (S32) reads and writes them as such.
+static int fused_grid_run_s32(const nls_problem *p, int iters, const float *P, float2 *psi, float *nres, char *ws,
+                              hipStream_t s) {
+    const size_t N = (size_t)p->n * p->n;
+    const bool dyn = p->coeffs[10] != 0.0;
+    const int form = fused_form(NLS_F64, p->order, p->bc, tiles_of(p->n, p->n, 16), tiles_of(p->n, p->n, 32));
+    const int ty = fused_ty(p->order, form);
+    FusedArgs<double> f;
+    memset(&f, 0, sizeof f);
+    f.grid_ntx = (p->n + 63) / 64;
+    f.ntiles = f.grid_ntx * ((p->n + ty - 1) / ty);
+    f.chunk = (f.ntiles + 7) / 8;
+    f.grid = plain_member<double>(p);
+    f.P = (const double *)P;
+    float2 *cur = psi, *alt = (float2 *)ws;
+    float *curn = nres, *altn = dyn ? (float *)(ws + align256b(N * sizeof(float2))) : nullptr;
+    for (int it = 0; it < iters; ++it) {
+        f.it = it;
+        f.u = (const double2 *)cur; f.un = (const double *)curn; f.uo = (double2 *)alt; f.uno = (double *)altn;
+        launch_fused_s32(p->order, dyn, form, f, s);
+        std::swap(cur, alt);
+        std::swap(curn, altn);
+    }
+    int rc = hip_check(hipGetLastError(), "fused step launch");
+    if (rc) return rc;
+    if (cur != psi) {
+        rc = hip_check(hipMemcpyAsync(psi, cur, N * sizeof(float2), hipMemcpyDeviceToDevice, s), "copy state");
+        if (rc) return rc;
+        if (dyn) rc = hip_check(hipMemcpyAsync(nres, curn, N * sizeof(float), hipMemcpyDeviceToDevice, s), "copy reservoir");
+    }
+    return rc;
+}
+
 int fused_grid_advance(const nls_problem *p, int iters, const void *pumping, void *psi, void *nres, void *ws,
                        hipStream_t s) {
+    if (p->flags & NLS_FLAG_F64_ARITH) {
+        // double-precision addressing: grids of fewer than 2^32 points
+        if (!fused_holds((long long)p->n * p->n, NLS_F64))
+            return fail(NLS_EUNSUPPORTED, "an %d x %d grid is too large for the fused kernel", p->n, p->n);
+        return fused_grid_run_s32(p, iters, (const float *)pumping, (float2 *)psi, (float *)nres, (char *)ws, s);
+    }
     if (!fused_holds((long long)p->n * p->n, p->dtype))
         return fail(NLS_EUNSUPPORTED, "an %d x %d grid is too large for the fused kernel (single precision: "
                     "complex array of 4 GiB or more; double: 2^32 points or more); the per-stage engine holds it",

@@ -29,6 +29,11 @@ int check_problem(const nls_problem *p) {
     if (!(p->dx > 0.0)) return fail(NLS_EINVAL, "dx must be positive");
     if (p->scheme == NLS_SCHEME_SSF && p->bc != NLS_BC_PERIODIC)
         return fail(NLS_EUNSUPPORTED, "the split-step Fourier scheme needs periodic boundaries");
+    if ((p->flags & NLS_FLAG_F64_ARITH) &&
+        (p->dtype != NLS_F32 || p->scheme != NLS_SCHEME_RK4 || p->dim != 2 || p->bc != NLS_BC_PERIODIC ||
+         (p->engine != NLS_ENGINE_AUTO && p->engine != NLS_ENGINE_FUSED)))
+        return fail(NLS_EUNSUPPORTED, "NLS_FLAG_F64_ARITH: single-precision 2-D periodic RK4 problems on the fused "
+                                      "engine");
     if (p->scheme == NLS_SCHEME_SSF) return ssf_check(p);
     if (p->engine == NLS_ENGINE_HYBRID)
         return fail(NLS_EINVAL, "NLS_ENGINE_HYBRID splits a batch between engines; a single instance takes one engine");
@@ -50,6 +55,7 @@ int check_problem(const nls_problem *p) {
 // kernel's tiles) -- else the fused one-launch-per-step kernel for 2-D grids
 // and the per-stage kernels for 1-D lines.
 int plain_engine(const nls_problem *p) {
+    if (p->flags & NLS_FLAG_F64_ARITH) return NLS_ENGINE_FUSED;
     if (p->engine != NLS_ENGINE_AUTO) return p->engine;
     if (resident_fits(p) && (p->dim == 1 || p->dtype == NLS_F32)) return NLS_ENGINE_RESIDENT;
     return p->dim == 2 && fused_holds((long long)p->n * p->n, p->dtype) ? NLS_ENGINE_FUSED : NLS_ENGINE_STAGE;
@@ -257,7 +263,8 @@ static int block_graph(const nls_problem *p, const void *P, void *psi, void *nre
     GraphCache &gc = *g_graphs;
     // a graph holds the kernels of the launch configuration it was captured
     // with: the per-stage tile height (nls_tune_stage_tile) is part of the key
-    const int ty = get_stage_tile(p->dtype), ff = get_fused_form(p->dtype);
+    const int ty = get_stage_tile(p->dtype),
+              ff = get_fused_form((p->flags & NLS_FLAG_F64_ARITH) ? NLS_F64 : p->dtype);
     for (GraphEntry &e : gc.entries)
         if (e.device == dev && e.stage_ty == ty && e.fform == ff && e.P == P && e.psi == psi && e.nres == nres &&
             e.ws == ws &&

@@ -42,6 +42,7 @@ EXPORTS = (
 )
 ENGINE = {"auto": -1, "stage": 0, "fused": 1, "resident": 2, "hybrid": 3}
 FLAG_EAGER = 1
+FLAG_F64_ARITH = 2
 
 
 class NativeError(RuntimeError):
@@ -67,19 +68,21 @@ class ProblemDesc(C.Structure):
 
     @classmethod
     def make(cls, dim, n, order, dt, dx, coeffs, bc="ref", dtype="f32", scheme="rk4", engine="auto",
-             graphs=True):
+             graphs=True, f64_arith=False):
         """engine: RK4 engine of a plain solve -- "auto" (resident when the
         instance fits on chip, else fused for 2-D and stage for 1-D),
         "stage" (one launch per RK4 stage), "fused" (one launch per step,
         2-D only) or "resident" (1-D n <= 4096, 2-D n <= 64).  graphs=False
-        sets NLS_FLAG_EAGER: no step-block graph replay."""
+        sets NLS_FLAG_EAGER: no step-block graph replay.  f64_arith=True
+        sets NLS_FLAG_F64_ARITH: a single-precision 2-D periodic problem
+        stepped with double-precision arithmetic on its float fields."""
         p = cls()
         p.dim, p.n, p.order = int(dim), int(n), int(order)
         p.bc = BC[bc] if isinstance(bc, str) else int(bc)
         p.dtype = DTYPE[dtype] if isinstance(dtype, str) else int(dtype)
         p.scheme = SCHEME[scheme] if isinstance(scheme, str) else int(scheme)
         p.engine = ENGINE[engine] if isinstance(engine, str) else int(engine)
-        p.flags = 0 if graphs else FLAG_EAGER
+        p.flags = (0 if graphs else FLAG_EAGER) | (FLAG_F64_ARITH if f64_arith else 0)
         p.dt, p.dx = float(dt), float(dx)
         c = np.zeros(23)
         cc = np.asarray(coeffs, np.float64).ravel()
