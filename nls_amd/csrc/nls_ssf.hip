@@ -71,6 +71,13 @@ template <> struct ColShape<double, true> { static constexpr int LOG_L = 2, PTS 
 #ifndef NLS_SSF_COL_PAD
 #define NLS_SSF_COL_PAD 4
 #endif
+// The 2048-point column pass (four columns, stages in registers) is the
+// faster unpadded: 2048^2 per step, same box, two rounds each, FP64 95.2 /
+// 94.8 -> 93.8 / 93.7 us, FP32 storage 87.8 / 87.6 -> 87.0 / 87.1 us (one pad
+// per 8 or per 32 complex: in between).
+#ifndef NLS_SSF_COL2048_PAD
+#define NLS_SSF_COL2048_PAD 31
+#endif
 #ifndef NLS_SSF_LINE_PAD
 #define NLS_SSF_LINE_PAD 4
 #endif
@@ -658,8 +665,9 @@ __global__ void __launch_bounds__((ColShape<T, COL2048>::NTH)) k_ssf_cols(const 
     constexpr int COL_PTS = CS::PTS, COL_NTH = CS::NTH;
     constexpr int PPT = COL_PTS / COL_NTH;
     static_assert(COL_PTS == 2048 << CS::LOG_L, "column block of 2048-point columns");
-    __shared__ C buf_mem[lds_padded(COL_PTS, NLS_SSF_COL_PAD)];
-    const Lds<T, NLS_SSF_COL_PAD> buf{buf_mem};
+    constexpr int PAD = COL2048 ? NLS_SSF_COL2048_PAD : NLS_SSF_COL_PAD;
+    __shared__ C buf_mem[lds_padded(COL_PTS, PAD)];
+    const Lds<T, PAD> buf{buf_mem};
     const int b = blockIdx.x;
     const int blk = (b & 7) * chunk + (b >> 3);
     if (blk >= nblocks) return;
