@@ -40,13 +40,22 @@ def workr_model(power, **overrides):
 
 
 class _Member(object):
-    """A model as a batch member (n, dx, dt, coeffs, P, psi0, n0)."""
+    """A model as a batch member (n, dx, dt, coeffs, P, psi0, n0).
+
+    A model in the reference's own mode (single precision, 'ref'
+    boundaries, RK4, algebraic reservoir) solves through the drop-ins,
+    whose dt, dx and coefficients are single precision (real(sp),
+    nls.f90:800-804); its member takes them rounded the same way, so that
+    it steps exactly as its model.solve() does."""
 
     def __init__(self, model):
         self.n = int(model.getNumberOfNodes())
         self.dx = float(model.getSpatialStep())
         self.dt = float(model.getTimeStep())
         self.coeffs = np.asarray(model.getCoefficients(), np.float64)
+        if model.getSolver()._reference_mode(self.coeffs):
+            self.dx, self.dt = float(np.float32(self.dx)), float(np.float32(self.dt))
+            self.coeffs = self.coeffs.astype(np.float32).astype(np.float64)
         self.P = np.asarray(model.getPumping())
         self.psi0 = np.asarray(model.getInitialSolution())
         self.n0 = model.getInitialReservoir()

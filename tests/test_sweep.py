@@ -79,3 +79,20 @@ def test_gpu_sweep_against_oracle(gpu, oracle):
                                  m.getInitialSolution(), np.zeros((n, n)))
         assert rel(s.getSolution(), opsi) <= 1e-10
         assert rel(s.getReservoir(), onres) <= 1e-10
+
+
+@pytest.mark.gpu
+def test_gpu_sweep_1d_radial_equals_each_solve(gpu):
+    """1-D radial models (the reference's default model family) of different
+    sizes, powers and step counts in one batch: each equals its own
+    model.solve() bit for bit (the resident engine alone and in the batch)."""
+    models = [M.Problem().model(model="1d", num_nodes=n, num_iters=it, dx=0.1, dt=1e-3,
+                                pumping=PU.GaussianRingPumping1D(power=p, radius=5.0, variation=3.14))
+              for n, p, it in ((200, 2.0, 300), (400, 5.0, 301), (1000, 20.0, 150))]
+    sols = sweep.solve_models(models)
+    for m, s in zip(models, sols):
+        alone = M.Problem().model(model="1d", num_nodes=m.getNumberOfNodes(), num_iters=m.getNumberOfIterations(),
+                                  dx=0.1, dt=1e-3, pumping=m.pumping).solve().getSolution()
+        assert s.getSolution().shape == (m.getNumberOfNodes(),)
+        assert np.isfinite(s.getSolution()).all()
+        np.testing.assert_array_equal(s.getSolution(), alone)
