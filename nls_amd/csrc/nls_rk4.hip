@@ -12,7 +12,7 @@ namespace nls {
 // Tiles are numbered row-major over the grid; workgroups reach the 8 XCDs
 // round-robin, so block b takes tile (b % 8) * chunk + b / 8 and every XCD
 // sweeps a contiguous band of tile rows (its halo rows stay in its L2).
-template <typename T, int ORDER, int BC, bool DYN, int TY>
+template <typename T, int ORDER, int BC, bool DYN, int TY, bool NT>
 __global__ void __launch_bounds__(256) k_rk4_stage_2d(const StageArgs<T> a, int ntx, int ntiles, int chunk) {
     constexpr int R = (ORDER - 1) / 2;
     __shared__ cplx_t<T> tile[(TY + 2 * R) * (TILE2_X + 2 * R)];
@@ -20,7 +20,7 @@ __global__ void __launch_bounds__(256) k_rk4_stage_2d(const StageArgs<T> a, int 
     const int t = (b & 7) * chunk + (b >> 3);
     if (t >= ntiles) return;
     const int ty = t / ntx, tx = t - ty * ntx;
-    stage_tile_2d<T, ORDER, BC, DYN, TY>(a, tx * TILE2_X, ty * TY, tile);
+    stage_tile_2d<T, ORDER, BC, DYN, TY, NT>(a, tx * TILE2_X, ty * TY, tile);
 }
 
 template <typename T, int ORDER, int BC, bool DYN>
@@ -43,6 +43,25 @@ __global__ void k_radial_table(int n, int order, double h, T *wtab) {
 // by nls_tune_stage_tile from any thread)
 static std::atomic<int> g_stage_ty[2] = {{16}, {16}};
 
+// Non-temporal stores of k and the accumulator pay once a stage's arrays
+// (state, pumping, two k buffers and the accumulator) exceed the 256 MB
+// last-level cache: 2048^2 FP64 dynamic (436 MB) 294.4 -> 262.0 us per step;
+// below it the next stage finds them in that cache, and the hint costs
+// (2048^2 FP32, 218 MB: 123.5 -> 136.6 us).
+constexpr double NT_STAGE_BYTES = 256.0 * 1024 * 1024;
+
+template <typename T, int ORDER, int BC, bool DYN, int TY>
+static void launch_stage_2d(const StageArgs<T> &a, int ntx, int ntiles, int chunk, hipStream_t s) {
+    const double pts = (double)a.n * a.n;
+    const double bytes = pts * (4.0 * sizeof(cplx_t<T>) + (DYN ? 5.0 : 1.0) * sizeof(T));
+    if (bytes > NT_STAGE_BYTES)
+        hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, TY, true>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
+                           ntiles, chunk);
+    else
+        hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, TY, false>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
+                           ntiles, chunk);
+}
+
 template <typename T, int ORDER, int BC, bool DYN>
 static void launch_stage(int dim, const StageArgs<T> &a, hipStream_t s) {
     if (dim == 2) {
@@ -50,18 +69,10 @@ static void launch_stage(int dim, const StageArgs<T> &a, hipStream_t s) {
         const int ntx = (a.n + TILE2_X - 1) / TILE2_X;
         const int nty = (a.n + ty - 1) / ty;
         const int ntiles = ntx * nty, chunk = (ntiles + 7) / 8;
-        if (ty == 4)
-            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 4>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
-                               ntiles, chunk);
-        else if (ty == 8)
-            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 8>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
-                               ntiles, chunk);
-        else if (ty == 16)
-            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 16>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
-                               ntiles, chunk);
-        else
-            hipLaunchKernelGGL((k_rk4_stage_2d<T, ORDER, BC, DYN, 32>), dim3(8 * chunk), dim3(256), 0, s, a, ntx,
-                               ntiles, chunk);
+        if (ty == 4) launch_stage_2d<T, ORDER, BC, DYN, 4>(a, ntx, ntiles, chunk, s);
+        else if (ty == 8) launch_stage_2d<T, ORDER, BC, DYN, 8>(a, ntx, ntiles, chunk, s);
+        else if (ty == 16) launch_stage_2d<T, ORDER, BC, DYN, 16>(a, ntx, ntiles, chunk, s);
+        else launch_stage_2d<T, ORDER, BC, DYN, 32>(a, ntx, ntiles, chunk, s);
     } else {
         dim3 grid((a.n + TILE1 - 1) / TILE1);
         hipLaunchKernelGGL((k_rk4_stage_1d<T, ORDER, BC, DYN>), grid, dim3(TILE1), 0, s, a);

@@ -48,8 +48,10 @@ constexpr int TILE1 = 256;
 // this stage unless it is the last one; the accumulator unless it is the
 // first); an if-chain with a store per stage let the compiler merge the
 // accumulator stores behind a base-address phi that it left undefined on one
-// path.
-template <typename T, bool DYN>
+// path.  NT: non-temporal stores, for grids whose per-stage arrays exceed the
+// last-level cache (the next stage reads them from HBM in any case; the hint
+// keeps them from evicting the lines the next tiles' halos re-read).
+template <typename T, bool DYN, bool NT = false>
 __device__ __forceinline__ void stage_epilogue(const StageArgs<T> &a, size_t gi, cplx_t<T> u,
                                                cplx_t<T> kin, T kin_n, cplx_t<T> acc, T accn, cplx_t<T> k, T kn,
                                                T un) {
@@ -68,13 +70,25 @@ __device__ __forceinline__ void stage_epilogue(const StageArgs<T> &a, size_t gi,
         av = mk<T>(sr, si);
         avn = sn;
     }
-    if (st != 4) {
-        a.kout[gi] = k;
-        if (DYN) a.kout_n[gi] = kn;
-    }
-    if (st != 1) {
-        a.acc[gi] = av;
-        if (DYN) a.acc_n[gi] = avn;
+    if constexpr (NT) {
+        using V2 = T __attribute__((ext_vector_type(2)));
+        if (st != 4) {
+            __builtin_nontemporal_store(V2{k.x, k.y}, reinterpret_cast<V2 *>(&a.kout[gi]));
+            if (DYN) __builtin_nontemporal_store(kn, &a.kout_n[gi]);
+        }
+        if (st != 1) {
+            __builtin_nontemporal_store(V2{av.x, av.y}, reinterpret_cast<V2 *>(&a.acc[gi]));
+            if (DYN) __builtin_nontemporal_store(avn, &a.acc_n[gi]);
+        }
+    } else {
+        if (st != 4) {
+            a.kout[gi] = k;
+            if (DYN) a.kout_n[gi] = kn;
+        }
+        if (st != 1) {
+            a.acc[gi] = av;
+            if (DYN) a.acc_n[gi] = avn;
+        }
     }
 }
 
@@ -120,7 +134,7 @@ __device__ __forceinline__ void stage_point(const StageArgs<T> &a, size_t gi, cp
 // points is issued before the LDS barrier, and the own points' stage inputs
 // double as the interior of the LDS tile, so u and k_prev are read once; only
 // the R-wide halo ring is loaded separately.  tile holds (TY+2R) x (64+2R).
-template <typename T, int ORDER, int BC, bool DYN, int TY>
+template <typename T, int ORDER, int BC, bool DYN, int TY, bool NT = false>
 __device__ __forceinline__ void stage_tile_2d(const StageArgs<T> &a, int x0, int y0, cplx_t<T> *tile) {
     using C = cplx_t<T>;
     constexpr int R = (ORDER - 1) / 2;
@@ -226,7 +240,7 @@ __device__ __forceinline__ void stage_tile_2d(const StageArgs<T> &a, int x0, int
         C k;
         T kn = 0;
         rhs_point<T, DYN>(a.c, pp[j], c, yn, mk<T>(lr, li), k, kn);
-        stage_epilogue<T, DYN>(a, gi[j], u[j], kin[j], kn_in[j], acc[j], accn[j], k, kn, un[j]);
+        stage_epilogue<T, DYN, NT>(a, gi[j], u[j], kin[j], kn_in[j], acc[j], accn[j], k, kn, un[j]);
     }
 }
 
