@@ -289,3 +289,25 @@ def test_unstable_dt_is_reported(gpu, dim):
     fn = native.nls.solve_nls if dim == 1 else native.nls.solve_nls_2d
     u = fn(dt, dx, 5, 400, P, configs.model_coeffs(dynamic=False), psi0)
     assert not np.all(np.isfinite(u))
+
+
+def test_stage_f32_nontemporal_grid(gpu, oracle):
+    """A single-precision grid past the per-stage kernels' non-temporal-store
+    threshold (2304^2 periodic, dynamic reservoir: 276 MB of stage arrays,
+    above 256 MB) for 4 steps on the per-stage engine, against the FP64
+    oracle (the FP64 form of that store path runs in
+    test_square2048_f64_first_steps)."""
+    n, dx, dt, steps = 2304, 0.0625, 2.5e-4, 4
+    x = -n * dx / 2 + dx * np.arange(n)
+    r = np.sqrt(x[:, None] ** 2 + x[None, :] ** 2)
+    P = configs.ring(r, 3.0, 20.0, 2.0)
+    c = configs.model_coeffs(dynamic=True)
+    psi0 = (0.1 * np.exp(-r ** 2 / 400.0) + configs.noise(np.random.default_rng(5), (n, n))).astype(np.complex128)
+    n0 = np.zeros((n, n))
+    oracle.set_threads(8)
+    opsi, onres = oracle.rk4(2, n, 5, 1, dt, dx, steps, P, c, psi0, n0)
+    desc = native.ProblemDesc.make(2, n, 5, dt, dx, c, bc="periodic", dtype="f32", engine="stage")
+    psi, nres = native.solve_host(desc, steps, P, psi0, n0)
+    assert np.all(np.isfinite(psi))
+    assert rel(psi, opsi) <= TOL32, rel(psi, opsi)
+    assert rel(nres, onres) <= TOL32, rel(nres, onres)
